@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Forward + adjoint NFFT throughput on one MI355X.
+
+Workload: 3D NFFT, N = 128^3, n = 256^3, m = 4, M = 2^21 nodes (half uniform in
+[-0.5,0.5)^3, half N(0, 0.1^2) per axis wrapped onto the torus), fhat and f
+re/im N(0,1).  One step = one forward NFFT plus one adjoint NFFT on inputs
+resident in HBM (nodes binned once, before timing; x is fixed across steps).
+The headline runs in FP32 complex with the FP64 index/fraction split and FP64
+roll-off factors; the same steps are timed in full FP64 beside it.
+
+Prints one JSON line (the last line of stdout).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N3 = [128, 128, 128]
+NN3 = [256, 256, 256]
+M_NODES = 2 ** 21
+CUT = 4
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(seed=20240603):
+    r = np.random.default_rng(seed)
+    half = M_NODES // 2
+    x = np.empty((M_NODES, 3))
+    x[:half] = r.uniform(-0.5, 0.5, (half, 3))
+    c = r.normal(0.0, 0.1, (M_NODES - half, 3))
+    x[half:] = c - np.floor(c + 0.5)          # wrap onto [-0.5, 0.5)
+    Nt = int(np.prod(N3))
+    fh = r.standard_normal(Nt) + 1j * r.standard_normal(Nt)
+    fv = r.standard_normal(M_NODES) + 1j * r.standard_normal(M_NODES)
+    return x, fh, fv
+
+
+def grid_bytes(cbytes):
+    """Algorithmic bytes of one gridding launch: the grid once plus the nodes."""
+    n_total = int(np.prod(NN3))
+    return cbytes * n_total + M_NODES * (3 * 8 + cbytes)
+
+
+def time_plan(cu, precision, x, fh, fv, steps, warmup):
+    import torch
+
+    p = cu.Plan(N3, M_NODES, n=NN3, m=CUT, precision=precision)
+    p.x[:] = x
+    p.f_hat[:] = fh
+    p.f[:] = fv
+    p.copyDataToDevice()
+    p.copyDataToDeviceAd()
+    dp = p.device_pointers
+    # warm-up (also builds the node binning once)
+    for _ in range(warmup):
+        p.transform_dev(dp["f_hat"], dp["f"])
+        p.adjoint_dev(dp["f"], dp["f_hat"])
+    torch.cuda.synchronize()
+    p.copyDataToDevice()
+    p.copyDataToDeviceAd()
+    torch.cuda.synchronize()
+    # timed region: K steps of forward + adjoint, device-resident data
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.transform_dev(dp["f_hat"], dp["f"])
+        p.adjoint_dev(dp["f"], dp["f_hat"])
+    torch.cuda.synchronize()  # device-wide: waits for the plan's own stream too
+    t_total = time.perf_counter() - t0
+    ms_per_step = 1e3 * t_total / steps
+    # per-step device times with HIP events on the plan stream
+    p._p.flags = p._p.flags | cu.MEASURE_TIME
+    fw_conv, fw_fft, ad_conv, ad_fft = [], [], [], []
+    for _ in range(max(3, min(steps, 10))):
+        p.transform()
+        t = p.times
+        fw_conv.append(t["time_CONV"])
+        fw_fft.append(t["time_FFT"])
+        p.adjoint()
+        t = p.times
+        ad_conv.append(t["time_CONV"])
+        ad_fft.append(t["time_FFT"])
+    p._p.flags = p._p.flags & ~cu.MEASURE_TIME
+    med = lambda v: float(np.median(v))
+    return p, {"ms_per_step": ms_per_step, "fwd_gather_ms": 1e3 * med(fw_conv),
+               "fwd_fft_ms": 1e3 * med(fw_fft), "adj_scatter_ms": 1e3 * med(ad_conv),
+               "adj_fft_ms": 1e3 * med(ad_fft)}
+
+
+def accuracy(cu, p32, p64, x, fh, n_sample=4096):
+    """Precision and window error on the bench inputs (GPU, untimed)."""
+    import torch
+
+    out = {}
+    r = np.random.default_rng(7)
+    rows = np.sort(r.choice(M_NODES, n_sample, replace=False)).astype(np.int64)
+    for p in (p32, p64):
+        p.copyDataToDevice()
+        p.transform()
+    f32 = p32.copy_f_ToHost().astype(np.complex128)
+    f64 = p64.copy_f_ToHost()
+    out["fwd_fp32_vs_fp64_rel_l2"] = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+    rows_t = torch.from_numpy(rows).cuda()
+    nd = torch.empty(n_sample, dtype=torch.complex128, device="cuda")
+    p64.ndft_rows_dev(p64.device_pointers["f_hat"], rows_t.data_ptr(), n_sample, nd.data_ptr())
+    torch.cuda.synchronize()
+    nd = nd.cpu().numpy()
+    out["fwd_fp64_vs_ndft_rel_l2_sample"] = float(np.linalg.norm(f64[rows] - nd) / np.linalg.norm(nd))
+    out["fwd_fp32_vs_ndft_rel_l2_sample"] = float(np.linalg.norm(f32[rows] - nd) / np.linalg.norm(nd))
+    for p in (p32, p64):
+        p.copyDataToDeviceAd()
+        p.adjoint()
+    h32 = p32.copy_f_hat_ToHost().astype(np.complex128)
+    h64 = p64.copy_f_hat_ToHost()
+    out["adj_fp32_vs_fp64_rel_l2"] = float(np.linalg.norm(h32 - h64) / np.linalg.norm(h64))
+    cols = np.sort(r.choice(int(np.prod(N3)), n_sample, replace=False)).astype(np.int64)
+    cols_t = torch.from_numpy(cols).cuda()
+    nh = torch.empty(n_sample, dtype=torch.complex128, device="cuda")
+    p64.ndft_cols_dev(p64.device_pointers["f"], cols_t.data_ptr(), n_sample, nh.data_ptr())
+    torch.cuda.synchronize()
+    nh = nh.cpu().numpy()
+    out["adj_fp64_vs_ndft_rel_l2_sample"] = float(np.linalg.norm(h64[cols] - nh) / np.linalg.norm(nh))
+    out["adj_fp32_vs_ndft_rel_l2_sample"] = float(np.linalg.norm(h32[cols] - nh) / np.linalg.norm(nh))
+    return out
+
+
+def cpu_baseline(x, fh, budget_s=15.0):
+    """The oracle's FP64 direct NDFT (OpenMP, all host cores) on sampled rows of
+    the same workload; throughput extrapolated linearly to the full fwd+adj NDFT."""
+    import oracle
+
+    oracle.build()
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, 16)
+    r = np.random.default_rng(11)
+    n_rows = 8
+    elapsed = 0.0
+    while True:
+        rows = r.choice(M_NODES, n_rows, replace=False).astype(np.int64)
+        t0 = time.perf_counter()
+        oracle.ndft_trafo(N3, x, fh, rows=rows, nthreads=cores)
+        elapsed = time.perf_counter() - t0
+        if elapsed > budget_s / 4 or n_rows >= 4096:
+            break
+        n_rows = int(n_rows * max(2.0, min(8.0, (budget_s / 4) / max(elapsed, 1e-3))))
+    terms = n_rows * int(np.prod(N3))
+    rate = terms / elapsed                              # terms per second
+    full_terms = 2.0 * M_NODES * int(np.prod(N3))       # forward + adjoint NDFT
+    value = M_NODES / (full_terms / rate)
+    return {"value": value, "unit": "nodes/s", "cores": int(cores), "kind": "port",
+            "sample": f"FP64 direct NDFT (oracle, OpenMP) of {n_rows} sampled output rows of the "
+                      f"forward transform ({terms:.3g} terms in {elapsed:.2f} s); fwd+adj rate "
+                      f"extrapolated linearly to the full 2*M*N_total terms"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the multi-GPU sharding is a design, not a run", file=sys.stderr)
+        return 2
+    import torch
+
+    import cunfft_amd as cu
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    log("bench: building inputs")
+    x, fh, fv = make_inputs()
+    log("bench: timing FP32")
+    p32, t32 = time_plan(cu, "single", x, fh, fv, a.steps, a.warmup)
+    log(f"bench: FP32 {t32}")
+    log("bench: timing FP64")
+    p64, t64 = time_plan(cu, "double", x, fh, fv, a.steps, a.warmup)
+    log(f"bench: FP64 {t64}")
+    log("bench: accuracy")
+    acc = accuracy(cu, p32, p64, x, fh)
+    log(f"bench: {acc}")
+    p32.finalize()
+    p64.finalize()
+
+    value = M_NODES / (t32["ms_per_step"] * 1e-3)
+    value64 = M_NODES / (t64["ms_per_step"] * 1e-3)
+    # dominant kernel of the FP32 step and its roofline
+    kern = max(("fwd_gather_ms", "adj_scatter_ms"), key=lambda k: t32[k])
+    ach = grid_bytes(8) / (t32[kern] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "gather (forward gridding)" if kern == "fwd_gather_ms"
+            else "scatter (adjoint gridding)", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("bench: CPU baseline")
+        cpu = cpu_baseline(x, fh)
+    line = {
+        "metric": "fwd+adj NFFT nodes/s, 3D 128^3 M=2^21, FP32 vs FP64 @<=1e-5 rel err; %HBM peak",
+        "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": t32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 complex; fp64 index split and roll-off)",
+        "data": "synthetic: half uniform, half N(0,0.1^2) wrapped nodes; N(0,1) coefficients",
+        "config": {"workload": "3D NFFT forward+adjoint, N=128^3, n=256^3, m=4, M=2^21 nodes",
+                   "N": N3, "n": NN3, "m": CUT, "M": M_NODES, "precision": "fp32"},
+        "fp64": {"value": value64, "ms_per_step": t64["ms_per_step"],
+                 "fp32_speedup_over_fp64": value / value64},
+        "steps_fp32_ms": t32, "steps_fp64_ms": t64, "accuracy": acc,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

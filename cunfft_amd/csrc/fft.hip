@@ -1,0 +1,270 @@
+// fft.hip -- LDS-resident Stockham FFT pass for gfx950 with fused roll-off.
+//
+// Replaces the three reference steps around the FFT:
+//   forward  kernelRollOf_{1,2,3}d + memset  (src/cunfft_kernel.cu:240-412,
+//            src/cunfft_wrapper.cu:32-75)  then cufftExec(.., CUFFT_FORWARD)
+//   adjoint  cufftExec(.., CUFFT_INVERSE) then kernelRollOf_adjoint_{1,2,3}d
+//            (src/cunfft_kernel.cu:264-467, src/cunfft_wrapper.cu:77-121)
+// A d-dimensional transform is d passes, one per axis.  Each workgroup loads
+// C whole lines of the axis into LDS, runs radix-4 (and one radix-2) Stockham
+// stages there, and writes the lines back: one HBM read and one write per
+// pass.  The forward passes skip lines that are still all zero and read only
+// the N_t nonzero entries of each line; the first forward pass reads fhat and
+// applies exp(fac k^2) on the way in.  The adjoint passes write only the N_t
+// entries that survive the crop; the last one writes fhat with exp(fac k^2).
+// Twiddles come from a per-length table computed in extended precision on
+// the host.
+#include "internal.hpp"
+
+namespace cunfft {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftMaxElems = 4096;  // C * L per workgroup
+constexpr int kMaxB4 = kFftMaxElems / 4 / kFftThreads;
+constexpr int kMaxB2 = kFftMaxElems / 2 / kFftThreads;
+constexpr int kMaxRowsPerGroup = 64;
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cmul(const cplx_t<T> a, const cplx_t<T> b)
+{
+    return cmake<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> twiddle(const cplx_t<T> *__restrict__ tw, int q, int sign)
+{
+    cplx_t<T> w = tw[q];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+// 4-point DFT with kernel exp(sign 2 pi i r q / 4).
+template <typename T> __device__ __forceinline__ void dft4(cplx_t<T> (&v)[4], int sign)
+{
+    const cplx_t<T> a0 = cmake<T>(v[0].x + v[2].x, v[0].y + v[2].y);
+    const cplx_t<T> a1 = cmake<T>(v[0].x - v[2].x, v[0].y - v[2].y);
+    const cplx_t<T> a2 = cmake<T>(v[1].x + v[3].x, v[1].y + v[3].y);
+    const cplx_t<T> a3 = cmake<T>(v[1].x - v[3].x, v[1].y - v[3].y);
+    // omega * a3 with omega = sign * i
+    const T sr = sign > 0 ? -a3.y : a3.y;
+    const T si = sign > 0 ? a3.x : -a3.x;
+    v[0] = cmake<T>(a0.x + a2.x, a0.y + a2.y);
+    v[2] = cmake<T>(a0.x - a2.x, a0.y - a2.y);
+    v[1] = cmake<T>(a1.x + sr, a1.y + si);
+    v[3] = cmake<T>(a1.x - sr, a1.y - si);
+}
+
+struct LineInfo {
+    int64_t gbase;  // grid offset of the line start (without the column)
+    int64_t fbase;  // fhat offset of the row (passes on the last axis)
+    double fac;     // product of roll-off factors of the axes before t
+    int valid;
+};
+
+// Map the line index pa (support of the axes before t, FFT order, last axis
+// fastest) to its grid row, its fhat row and its roll-off factor.
+template <typename T>
+__device__ LineInfo line_info(const FftPass<T> &P, int64_t pa)
+{
+    LineInfo li;
+    li.valid = pa < P.n_pa;
+    int64_t rem = li.valid ? pa : 0;
+    int64_t a = 0, gmul = 1, fq = 0, fmul = 1;
+    double fac = 1.0;
+    for (int tt = P.t - 1; tt >= 0; --tt) {
+        const int64_t Nn = P.N[tt], hN = Nn >> 1;
+        const int64_t p = rem % Nn;
+        rem /= Nn;
+        const int64_t i = p < hN ? p : p + (P.n[tt] - Nn);
+        const int64_t q = p < hN ? p + hN : p - hN;
+        a += i * gmul;
+        gmul *= P.n[tt];
+        fq += q * fmul;
+        fmul *= Nn;
+        fac *= P.roc[P.rocoff[tt] + p];
+    }
+    li.gbase = a * (int64_t)P.L * P.S;
+    li.fbase = fq * P.N[P.t];
+    li.fac = fac;
+    return li;
+}
+
+template <typename T>
+__device__ __forceinline__ void stage4(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int logL,
+                                       int C, int ldsL, int logNs, int sign, int tid)
+{
+    const int logQ = logL - 2;
+    const int Q = 1 << logQ;
+    const int Ns = 1 << logNs;
+    const int nb = C << logQ;
+    cplx_t<T> v[kMaxB4][4];
+#pragma unroll
+    for (int k = 0; k < kMaxB4; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            const int c = b >> logQ, j = b & (Q - 1);
+            const cplx_t<T> *base = buf + c * ldsL + j;
+            v[k][0] = base[0];
+            v[k][1] = base[Q];
+            v[k][2] = base[2 * Q];
+            v[k][3] = base[3 * Q];
+            if (Ns > 1) {
+                const int step = (j & (Ns - 1)) << (logL - 2 - logNs);
+                v[k][1] = cmul<T>(v[k][1], twiddle<T>(tw, step, sign));
+                v[k][2] = cmul<T>(v[k][2], twiddle<T>(tw, 2 * step, sign));
+                v[k][3] = cmul<T>(v[k][3], twiddle<T>(tw, 3 * step, sign));
+            }
+            dft4<T>(v[k], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kMaxB4; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            const int c = b >> logQ, j = b & (Q - 1);
+            const int out = ((j >> logNs) << (logNs + 2)) + (j & (Ns - 1));
+            cplx_t<T> *ob = buf + c * ldsL + out;
+            ob[0] = v[k][0];
+            ob[Ns] = v[k][1];
+            ob[2 * Ns] = v[k][2];
+            ob[3 * Ns] = v[k][3];
+        }
+    }
+    __syncthreads();
+}
+
+// Final radix-2 stage (Ns = L/2): butterfly j combines j and j + L/2 in place.
+template <typename T>
+__device__ __forceinline__ void stage2_last(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw,
+                                            int logL, int C, int ldsL, int sign, int tid)
+{
+    const int logH = logL - 1;
+    const int H = 1 << logH;
+    const int nb = C << logH;
+#pragma unroll
+    for (int k = 0; k < kMaxB2; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            const int c = b >> logH, j = b & (H - 1);
+            cplx_t<T> *base = buf + c * ldsL + j;
+            const cplx_t<T> v0 = base[0];
+            const cplx_t<T> v1 = cmul<T>(base[H], twiddle<T>(tw, j, sign));
+            base[0] = cmake<T>(v0.x + v1.x, v0.y + v1.y);
+            base[H] = cmake<T>(v0.x - v1.x, v0.y - v1.y);
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
+{
+    using C2 = cplx_t<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2 *buf = reinterpret_cast<C2 *>(smem_raw);
+    __shared__ int64_t s_gbase[kMaxRowsPerGroup];
+    __shared__ int64_t s_fbase[kMaxRowsPerGroup];
+    __shared__ double s_fac[kMaxRowsPerGroup];
+    __shared__ int s_valid[kMaxRowsPerGroup];
+
+    const int tid = threadIdx.x;
+    const int L = P.L, logL = P.logL, C = P.C, logC = P.logC, ldsL = P.ldsL;
+    const bool rows = (P.S == 1);
+    int64_t pa0, s0;
+    if (rows) {
+        pa0 = (int64_t)blockIdx.x << logC;
+        s0 = 0;
+    } else {
+        const int64_t nch = P.S >> logC;
+        pa0 = (int64_t)blockIdx.x / nch;
+        s0 = ((int64_t)blockIdx.x - pa0 * nch) << logC;
+    }
+    if (tid < (rows ? C : 1)) {
+        const LineInfo li = line_info<T>(P, pa0 + tid);
+        s_gbase[tid] = li.gbase;
+        s_fbase[tid] = li.fbase;
+        s_fac[tid] = li.fac;
+        s_valid[tid] = li.valid;
+    }
+    __syncthreads();
+
+    const int total = C << logL;
+    const int Nt = (int)P.N[P.t];
+    const int hN = Nt >> 1;
+    const double *roct = P.roc + P.rocoff[P.t];
+
+    for (int e = tid; e < total; e += kFftThreads) {
+        int c, i;
+        if (rows) {
+            c = e >> logL;
+            i = e & (L - 1);
+        } else {
+            c = e & (C - 1);
+            i = e >> logC;
+        }
+        const int li = rows ? c : 0;
+        C2 v = cmake<T>(0, 0);
+        if (s_valid[li]) {
+            const bool insupp = (i < hN) || (i >= L - hN);
+            if (P.mode & FFT_FROM_FHAT) {
+                if (insupp) {
+                    const int p = i < hN ? i : i - (L - Nt);
+                    const int q = p < hN ? p + hN : p - hN;
+                    const C2 a = P.fhat_in[s_fbase[li] + q];
+                    const double f = s_fac[li] * roct[p];
+                    v = cmake<T>((T)((double)a.x * f), (T)((double)a.y * f));
+                }
+            } else if (!(P.mode & FFT_IN_PRUNED) || insupp) {
+                v = P.g[s_gbase[li] + (int64_t)i * P.S + s0 + (rows ? 0 : c)];
+            }
+        }
+        buf[c * ldsL + i] = v;
+    }
+    __syncthreads();
+
+    int logNs = 0;
+    while (logNs + 2 <= logL) {
+        stage4<T>(buf, P.tw, logL, C, ldsL, logNs, P.sign, tid);
+        logNs += 2;
+    }
+    if (logNs < logL) stage2_last<T>(buf, P.tw, logL, C, ldsL, P.sign, tid);
+
+    for (int e = tid; e < total; e += kFftThreads) {
+        int c, i;
+        if (rows) {
+            c = e >> logL;
+            i = e & (L - 1);
+        } else {
+            c = e & (C - 1);
+            i = e >> logC;
+        }
+        const int li = rows ? c : 0;
+        if (!s_valid[li]) continue;
+        const bool insupp = (i < hN) || (i >= L - hN);
+        if (P.mode & FFT_TO_FHAT) {
+            if (insupp) {
+                const int p = i < hN ? i : i - (L - Nt);
+                const int q = p < hN ? p + hN : p - hN;
+                const C2 v = buf[c * ldsL + i];
+                const double f = s_fac[li] * roct[p];
+                P.fhat_out[s_fbase[li] + q] = cmake<T>((T)((double)v.x * f), (T)((double)v.y * f));
+            }
+        } else if (!(P.mode & FFT_OUT_PRUNED) || insupp) {
+            P.g[s_gbase[li] + (int64_t)i * P.S + s0 + (rows ? 0 : c)] = buf[c * ldsL + i];
+        }
+    }
+}
+
+template <typename T>
+void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipStream_t s)
+{
+    if (blocks <= 0) return;
+    k_fft_pass<T><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
+    CUNFFT_HIP(hipGetLastError());
+}
+
+template void launch_fft_pass<float>(const FftPass<float> &, int64_t, size_t, hipStream_t);
+template void launch_fft_pass<double>(const FftPass<double> &, int64_t, size_t, hipStream_t);
+
+}  // namespace cunfft

@@ -1,0 +1,1005 @@
+// plan.hip -- plan lifecycle, step orchestration and the C ABI of cunfft.h.
+//
+// Mirrors the reference plan surface (src/cunfft.h:35-261, src/cunfft.cu:60-444)
+// with per-plan device state, one HIP stream per plan, runtime m and
+// precision, status codes instead of exit(), and the additive device-pointer
+// entry points.  Step order:
+//   cunfft_transform  (src/cunfft.cu:196-207): FFT passes with the roll-off/pad
+//                     fused into the first pass, then the gather.
+//   cunfft_adjoint    (src/cunfft.cu:209-224): scatter, then FFT passes with the
+//                     crop/roll-off fused into the last pass.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "cunfft.h"
+#include "internal.hpp"
+#include "bin.hpp"
+
+namespace cunfft {
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? CUNFFT_ERR_ALLOC : CUNFFT_ERR_HIP;
+        (void)hipGetLastError();
+        throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+static thread_local std::string g_last_message;
+
+template <typename T> struct Traits;
+template <> struct Traits<double> {
+    using plan = cunfft_plan;
+    using hc = cunfftDoubleComplex;
+};
+template <> struct Traits<float> {
+    using plan = cunfftf_plan;
+    using hc = cunfftFloatComplex;
+};
+
+// ------------------------------------------------------------ FFT tables --
+// Target of a cunfftHandle: twiddle tables tw[q] = exp(-2 pi i q / n_t) per axis.
+struct FftTables {
+    int prec;
+    int d;
+    int64_t n[3];
+    void *tw[3];
+};
+
+static FftTables *make_fft_tables(int prec, int d, const int64_t *n)
+{
+    FftTables *ft = new FftTables();
+    ft->prec = prec;
+    ft->d = d;
+    for (int t = 0; t < 3; ++t) {
+        ft->n[t] = t < d ? n[t] : 1;
+        ft->tw[t] = nullptr;
+    }
+    try {
+        for (int t = 0; t < d; ++t) {
+            const int64_t L = n[t];
+            const size_t es = prec == 64 ? sizeof(double2) : sizeof(float2);
+            std::vector<unsigned char> host((size_t)L * es);
+            for (int64_t q = 0; q < L; ++q) {
+                // angle reduced to the first octant symmetry-free form in long double
+                const long double ang = -2.0L * 3.141592653589793238462643383279502884L *
+                                        (long double)q / (long double)L;
+                const long double c = cosl(ang), s = sinl(ang);
+                if (prec == 64) {
+                    double2 v;
+                    v.x = (double)c;
+                    v.y = (double)s;
+                    memcpy(host.data() + q * es, &v, es);
+                } else {
+                    float2 v;
+                    v.x = (float)c;
+                    v.y = (float)s;
+                    memcpy(host.data() + q * es, &v, es);
+                }
+            }
+            CUNFFT_HIP(hipMalloc(&ft->tw[t], host.size()));
+            CUNFFT_HIP(hipMemcpy(ft->tw[t], host.data(), host.size(), hipMemcpyHostToDevice));
+        }
+    } catch (...) {
+        for (int t = 0; t < 3; ++t)
+            if (ft->tw[t]) (void)hipFree(ft->tw[t]);
+        delete ft;
+        throw;
+    }
+    return ft;
+}
+
+static void free_fft_tables(cunfftHandle h)
+{
+    FftTables *ft = static_cast<FftTables *>(h);
+    if (!ft) return;
+    for (int t = 0; t < 3; ++t)
+        if (ft->tw[t]) (void)hipFree(ft->tw[t]);
+    delete ft;
+}
+
+// ----------------------------------------------------------------- impl --
+template <typename T> struct Impl {
+    hipStream_t stream = nullptr;
+    Geom geom{};
+    double *roc = nullptr;  // device roll-off tables, N_t entries per axis
+    int64_t rocoff[3] = {0, 0, 0};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    Binning bins;           // node binning for the tiled gridding kernels
+    bool nodes_dirty = true;
+};
+
+template <typename T> static Impl<T> *impl_of(typename Traits<T>::plan *p)
+{
+    if (!p->impl) throw Error(CUNFFT_ERR_STATE, "plan is not initialised or was finalised");
+    return static_cast<Impl<T> *>(p->impl);
+}
+
+static int64_t next_pow2(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+static bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+static int ilog2(int64_t v)
+{
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+
+template <typename T> static void validate(const typename Traits<T>::plan *p)
+{
+    if (p->d < 1 || p->d > 3) throw Error(CUNFFT_ERR_INVALID_ARG, "d must be 1, 2 or 3");
+    if (p->m < 1 || p->m > CUNFFT_MAX_CUT_OFF)
+        throw Error(CUNFFT_ERR_INVALID_ARG, "cut-off m must be in [1, 16]");
+    if (p->M_total < 1) throw Error(CUNFFT_ERR_INVALID_ARG, "M_total must be >= 1");
+    if (!p->N || !p->n) throw Error(CUNFFT_ERR_INVALID_ARG, "N and n must not be NULL");
+    for (int t = 0; t < p->d; ++t) {
+        if (p->N[t] < 2 || (p->N[t] & 1))
+            throw Error(CUNFFT_ERR_INVALID_ARG, "N_t must be even and >= 2");
+        if (!is_pow2(p->n[t]))
+            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t must be a power of two");
+        if (p->n[t] < p->N[t]) throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= N_t");
+        if (p->n[t] < 2 * p->m + 2)
+            throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= 2m+2 (window wider than grid)");
+        if (p->n[t] > CUNFFT_MAX_AXIS)
+            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t above 4096 is not supported");
+    }
+}
+
+template <typename T> static void zero_fields(typename Traits<T>::plan *p)
+{
+    memset((void *)p, 0, sizeof(*p));
+}
+
+template <typename T> static void free_device(typename Traits<T>::plan *p)
+{
+    if (p->g_gpu) (void)hipFree(p->g_gpu);
+    if (p->fhat_gpu) (void)hipFree(p->fhat_gpu);
+    if (p->x_gpu) (void)hipFree(p->x_gpu);
+    if (p->f_gpu) (void)hipFree(p->f_gpu);
+    p->g_gpu = nullptr;
+    p->fhat_gpu = nullptr;
+    p->x_gpu = nullptr;
+    p->f_gpu = nullptr;
+}
+
+// src/cunfft.cu:262-280: allocate and zero the device arrays.
+template <typename T> static void init_device_para(typename Traits<T>::plan *p)
+{
+    using C2 = typename Traits<T>::hc;
+    free_device<T>(p);
+    CUNFFT_HIP(hipMalloc((void **)&p->g_gpu, sizeof(C2) * p->n_total));
+    CUNFFT_HIP(hipMalloc((void **)&p->fhat_gpu, sizeof(C2) * p->N_total));
+    CUNFFT_HIP(hipMalloc((void **)&p->x_gpu, sizeof(double) * p->M_total * p->d));
+    CUNFFT_HIP(hipMalloc((void **)&p->f_gpu, sizeof(C2) * p->M_total));
+    CUNFFT_HIP(hipMemset(p->g_gpu, 0, sizeof(C2) * p->n_total));
+    CUNFFT_HIP(hipMemset(p->fhat_gpu, 0, sizeof(C2) * p->N_total));
+    CUNFFT_HIP(hipMemset(p->x_gpu, 0, sizeof(double) * p->M_total * p->d));
+    CUNFFT_HIP(hipMemset(p->f_gpu, 0, sizeof(C2) * p->M_total));
+    if (p->impl) static_cast<Impl<T> *>(p->impl)->nodes_dirty = true;
+}
+
+// src/cunfft.cu:60-76: exp(-l^2 / b) for l = 0 .. 2m+1 (evaluated directly).
+static void fg_exp_l(double *out, double b, int offset, int m)
+{
+    for (int l = 0; l < 2 * m + 2; ++l) out[offset + l] = exp(-(double)l * (double)l / b);
+}
+
+// src/cunfft.cu:334-378 (cunfft_initHelp) on the fields set by init/initGuru.
+template <typename T> static void init_help(typename Traits<T>::plan *p)
+{
+    using C2 = typename Traits<T>::hc;
+    validate<T>(p);
+    const int d = p->d;
+    p->N_total = 1;
+    p->n_total = 1;
+    for (int t = 0; t < d; ++t) {
+        p->N_total *= p->N[t];
+        p->n_total *= p->n[t];
+    }
+    p->sigma = (double *)calloc(d, sizeof(double));
+    p->b = (double *)calloc(d, sizeof(double));
+    p->fg_exp_l_cpu = (double *)calloc((size_t)(2 * p->m + 2) * CUNFFT_MAX_DIM, sizeof(double));
+    if (!p->sigma || !p->b || !p->fg_exp_l_cpu) throw Error(CUNFFT_ERR_ALLOC, "host allocation");
+    for (int t = 0; t < d; ++t) {
+        p->sigma[t] = (double)p->n[t] / (double)p->N[t];
+        p->b[t] = (2.0 * p->sigma[t]) / (2.0 * p->sigma[t] - 1.0) * ((double)p->m / M_PI);
+        // per-axis table (the reference reuses axis 0's table for every axis,
+        // src/cunfft_kernel.cu:504)
+        fg_exp_l(p->fg_exp_l_cpu, p->b[t], t * (2 * p->m + 2), p->m);
+    }
+    memset(&p->CUNFFTTimes, 0, sizeof(p->CUNFFTTimes));
+    p->x = nullptr;
+    p->f_hat = nullptr;
+    p->f = nullptr;
+    p->g = nullptr;
+    p->g_hat = nullptr;
+    p->g_gpu = p->fhat_gpu = p->f_gpu = nullptr;
+    p->x_gpu = nullptr;
+    p->fft_forward = p->fft_backward = nullptr;
+    p->impl = nullptr;
+    if (!(p->flags & CUNFFT_NO_HOST_BUFFERS)) {
+        CUNFFT_HIP(hipHostMalloc((void **)&p->x, sizeof(double) * p->M_total * d, hipHostMallocDefault));
+        CUNFFT_HIP(hipHostMalloc((void **)&p->f_hat, sizeof(C2) * p->N_total, hipHostMallocDefault));
+        CUNFFT_HIP(hipHostMalloc((void **)&p->f, sizeof(C2) * p->M_total, hipHostMallocDefault));
+        CUNFFT_HIP(hipHostMalloc((void **)&p->g, sizeof(C2) * p->n_total, hipHostMallocDefault));
+        memset(p->x, 0, sizeof(double) * p->M_total * d);
+        memset(p->f_hat, 0, sizeof(C2) * p->N_total);
+        memset(p->f, 0, sizeof(C2) * p->M_total);
+        memset(p->g, 0, sizeof(C2) * p->n_total);
+    }
+    Impl<T> *im = new Impl<T>();
+    p->impl = im;
+    CUNFFT_HIP(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 4; ++k) CUNFFT_HIP(hipEventCreate(&im->ev[k]));
+
+    Geom &ge = im->geom;
+    ge.d = d;
+    ge.m = p->m;
+    ge.W = 2 * p->m + 2;
+    ge.M = p->M_total;
+    for (int t = 0; t < 3; ++t) {
+        ge.N[t] = t < d ? p->N[t] : 1;
+        ge.n[t] = t < d ? p->n[t] : 1;
+        ge.invb[t] = t < d ? 1.0 / p->b[t] : 0.0;
+        ge.nrm[t] = t < d ? 1.0 / sqrt(M_PI * p->b[t]) : 0.0;
+    }
+    // roll-off tables by pruned index p: exp(fac_t k^2), k = p or p - N_t
+    std::vector<double> roc;
+    for (int t = 0; t < d; ++t) {
+        im->rocoff[t] = (int64_t)roc.size();
+        const double fac = M_PI * M_PI * p->b[t] / ((double)p->n[t] * (double)p->n[t]);
+        const int64_t Nn = p->N[t];
+        for (int64_t q = 0; q < Nn; ++q) {
+            const int64_t k = q < Nn / 2 ? q : q - Nn;
+            roc.push_back(exp(fac * (double)k * (double)k));
+        }
+    }
+    CUNFFT_HIP(hipMalloc((void **)&im->roc, sizeof(double) * roc.size()));
+    CUNFFT_HIP(hipMemcpy(im->roc, roc.data(), sizeof(double) * roc.size(), hipMemcpyHostToDevice));
+    init_device_para<T>(p);
+    int64_t nn[3];
+    for (int t = 0; t < d; ++t) nn[t] = p->n[t];
+    const int prec = sizeof(T) == 8 ? 64 : 32;
+    if (p->flags & CUFFT_INIT) {
+        if (p->flags & (CUFFT_BOTH | CUFFT_FORW)) p->fft_forward = make_fft_tables(prec, d, nn);
+        if (p->flags & (CUFFT_BOTH | CUFFT_BACKW)) p->fft_backward = make_fft_tables(prec, d, nn);
+    }
+}
+
+template <typename T> static void finalize(typename Traits<T>::plan *p)
+{
+    Impl<T> *im = static_cast<Impl<T> *>(p->impl);
+    if (im && im->stream) (void)hipStreamSynchronize(im->stream);
+    free_fft_tables(p->fft_forward);
+    free_fft_tables(p->fft_backward);
+    p->fft_forward = p->fft_backward = nullptr;
+    if (p->g) (void)hipHostFree(p->g);
+    if (p->f) (void)hipHostFree(p->f);
+    if (p->x) (void)hipHostFree(p->x);
+    if (p->f_hat) (void)hipHostFree(p->f_hat);
+    p->g = p->f = p->f_hat = nullptr;
+    p->x = nullptr;
+    free(p->sigma);
+    free(p->n);
+    free(p->N);
+    free(p->b);
+    free(p->fg_exp_l_cpu);
+    p->sigma = p->b = p->fg_exp_l_cpu = nullptr;
+    p->n = p->N = nullptr;
+    free_device<T>(p);
+    if (im) {
+        im->bins.release();
+        if (im->roc) (void)hipFree(im->roc);
+        for (int k = 0; k < 4; ++k)
+            if (im->ev[k]) (void)hipEventDestroy(im->ev[k]);
+        if (im->stream) (void)hipStreamDestroy(im->stream);
+        delete im;
+    }
+    p->impl = nullptr;
+    p->status = CUNFFT_SUCCESS;
+}
+
+// ------------------------------------------------------------ FFT passes --
+template <typename T>
+static void fft_passes(typename Traits<T>::plan *p, Impl<T> *im, FftTables *ft, bool forward,
+                       const cplx_t<T> *fhat_in, cplx_t<T> *fhat_out, cplx_t<T> *g, hipStream_t s)
+{
+    const int d = p->d;
+    for (int k = 0; k < d; ++k) {
+        const int t = forward ? d - 1 - k : k;
+        FftPass<T> P{};
+        P.g = g;
+        P.fhat_in = fhat_in;
+        P.fhat_out = fhat_out;
+        P.tw = static_cast<const cplx_t<T> *>(ft->tw[t]);
+        P.roc = im->roc;
+        for (int q = 0; q < 3; ++q) {
+            P.rocoff[q] = im->rocoff[q];
+            P.n[q] = im->geom.n[q];
+            P.N[q] = im->geom.N[q];
+        }
+        P.d = d;
+        P.t = t;
+        P.L = (int)p->n[t];
+        P.logL = ilog2(P.L);
+        P.S = 1;
+        for (int q = t + 1; q < d; ++q) P.S *= p->n[q];
+        P.n_pa = 1;
+        for (int q = 0; q < t; ++q) P.n_pa *= p->N[q];
+        P.sign = forward ? -1 : +1;
+        if (forward)
+            P.mode = (t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED;
+        else
+            P.mode = (t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED;
+        const int64_t target = 2048;
+        int64_t C;
+        int64_t blocks;
+        if (P.S == 1) {
+            C = P.L >= target ? 1 : target / P.L;
+            C = std::min<int64_t>(C, 64);
+            C = std::min<int64_t>(C, next_pow2(P.n_pa));
+            blocks = (P.n_pa + C - 1) / C;
+            P.ldsL = P.L;
+        } else {
+            const int64_t cdef = sizeof(T) == 8 ? 8 : 16;
+            C = std::min<int64_t>(cdef, P.S);
+            C = std::min<int64_t>(C, std::max<int64_t>(1, 4096 / P.L));
+            blocks = P.n_pa * (P.S / C);
+            P.ldsL = C > 1 ? P.L + 1 : P.L;
+        }
+        P.C = (int)C;
+        P.logC = ilog2(C);
+        const size_t lds = (size_t)C * P.ldsL * sizeof(cplx_t<T>);
+        launch_fft_pass<T>(P, blocks, lds, s);
+    }
+}
+
+template <typename T> static FftTables *fft_tables(typename Traits<T>::plan *p, bool forward)
+{
+    cunfftHandle &h = forward ? p->fft_forward : p->fft_backward;
+    if (!h) {
+        int64_t nn[3];
+        for (int t = 0; t < p->d; ++t) nn[t] = p->n[t];
+        h = make_fft_tables(sizeof(T) == 8 ? 64 : 32, p->d, nn);
+    }
+    return static_cast<FftTables *>(h);
+}
+
+template <typename T>
+static void ensure_bins(typename Traits<T>::plan *p, Impl<T> *im, const double *x, hipStream_t s)
+{
+    if (!im->nodes_dirty) return;
+    im->bins.build(im->geom, x, s);
+    im->nodes_dirty = false;
+}
+
+// f = A fhat.  Returns the event-timed (fft, conv) seconds when timing.
+template <typename T>
+static void run_forward(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T> *fhat,
+                        cplx_t<T> *f, hipStream_t s, bool timing, double *t_fft, double *t_conv)
+{
+    cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
+    const double *x = p->x_gpu;
+    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+    fft_passes<T>(p, im, fft_tables<T>(p, true), true, fhat, nullptr, g, s);
+    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], s));
+    if (p->flags & CUNFFT_PLAIN_GATHER) {
+        launch_gather_plain<T>(im->geom, g, x, f, s);
+    } else {
+        ensure_bins<T>(p, im, x, s);
+        launch_gather_binned<T>(im->geom, im->bins, g, x, f, s);
+    }
+    if (timing) {
+        CUNFFT_HIP(hipEventRecord(im->ev[2], s));
+        CUNFFT_HIP(hipEventSynchronize(im->ev[2]));
+        float a = 0, b = 0;
+        CUNFFT_HIP(hipEventElapsedTime(&a, im->ev[0], im->ev[1]));
+        CUNFFT_HIP(hipEventElapsedTime(&b, im->ev[1], im->ev[2]));
+        *t_fft = a * 1e-3;
+        *t_conv = b * 1e-3;
+    }
+}
+
+// fhat = A^H f.
+template <typename T>
+static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T> *f,
+                        cplx_t<T> *fhat, hipStream_t s, bool timing, double *t_fft,
+                        double *t_conv)
+{
+    cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
+    const double *x = p->x_gpu;
+    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+    if (p->flags & CUNFFT_ATOMIC_SCATTER) {
+        launch_zero<T>(g, p->n_total, s);
+        launch_scatter_atomic<T>(im->geom, x, f, g, s);
+    } else {
+        ensure_bins<T>(p, im, x, s);
+        launch_scatter_binned<T>(im->geom, im->bins, x, f, g, s);
+    }
+    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], s));
+    fft_passes<T>(p, im, fft_tables<T>(p, false), false, nullptr, fhat, g, s);
+    if (timing) {
+        CUNFFT_HIP(hipEventRecord(im->ev[2], s));
+        CUNFFT_HIP(hipEventSynchronize(im->ev[2]));
+        float a = 0, b = 0;
+        CUNFFT_HIP(hipEventElapsedTime(&a, im->ev[0], im->ev[1]));
+        CUNFFT_HIP(hipEventElapsedTime(&b, im->ev[1], im->ev[2]));
+        *t_conv = a * 1e-3;
+        *t_fft = b * 1e-3;
+    }
+}
+
+template <typename T>
+static double timed_copy(Impl<T> *im, bool timing, void *dst, const void *src, size_t bytes,
+                         hipMemcpyKind kind)
+{
+    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], im->stream));
+    CUNFFT_HIP(hipMemcpyAsync(dst, src, bytes, kind, im->stream));
+    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], im->stream));
+    CUNFFT_HIP(hipStreamSynchronize(im->stream));
+    if (!timing) return 0.0;
+    float ms = 0;
+    CUNFFT_HIP(hipEventElapsedTime(&ms, im->ev[0], im->ev[1]));
+    return ms * 1e-3;
+}
+
+template <typename T> static void need_host(const typename Traits<T>::plan *p)
+{
+    if (!p->x || !p->f || !p->f_hat || !p->g)
+        throw Error(CUNFFT_ERR_STATE, "plan was created with CUNFFT_NO_HOST_BUFFERS");
+}
+
+template <typename T> static void need_device(const typename Traits<T>::plan *p)
+{
+    if (!p->g_gpu || !p->fhat_gpu || !p->x_gpu || !p->f_gpu)
+        throw Error(CUNFFT_ERR_STATE, "device arrays were released (cunfft_finalizeDevice)");
+}
+
+// ------------------------------------------------------- guarded calls --
+static void report(const char *what)
+{
+    g_last_message = what;
+    if (getenv("CUNFFT_VERBOSE")) fprintf(stderr, "cunfft: %s\n", what);
+}
+
+template <typename P, typename F> static int guard(P *p, F &&fn)
+{
+    if (!p) return CUNFFT_ERR_INVALID_ARG;
+    if (p->status != CUNFFT_SUCCESS) return p->status;
+    try {
+        fn();
+    } catch (const Error &e) {
+        p->status = e.code;
+        report(e.what());
+    } catch (const std::bad_alloc &) {
+        p->status = CUNFFT_ERR_ALLOC;
+        report("host allocation failed");
+    } catch (const std::exception &e) {
+        p->status = CUNFFT_ERR_HIP;
+        report(e.what());
+    }
+    return p->status;
+}
+
+template <typename T>
+static void api_init_guru(typename Traits<T>::plan *p, int dim, const uint_t *N, uint_t M,
+                          const uint_t *n, int m, unsigned flags, bool default_n)
+{
+    if (!p) return;
+    zero_fields<T>(p);
+    guard(p, [&] {
+        if (dim < 1 || dim > 3) throw Error(CUNFFT_ERR_INVALID_ARG, "d must be 1, 2 or 3");
+        if (!N) throw Error(CUNFFT_ERR_INVALID_ARG, "N is NULL");
+        if (!default_n && !n) throw Error(CUNFFT_ERR_INVALID_ARG, "n is NULL");
+        p->d = dim;
+        p->N = (uint_t *)malloc(sizeof(uint_t) * dim);
+        p->n = (uint_t *)malloc(sizeof(uint_t) * dim);
+        if (!p->N || !p->n) throw Error(CUNFFT_ERR_ALLOC, "host allocation");
+        for (int t = 0; t < dim; ++t) {
+            p->N[t] = N[t];
+            // src/cunfft.cu:296-299: n_t = 2 * next_pow2(N_t)
+            p->n[t] = default_n ? 2 * next_pow2(N[t]) : n[t];
+        }
+        p->M_total = M;
+        p->m = m;
+        p->flags = flags;
+        init_help<T>(p);
+    });
+}
+
+}  // namespace cunfft
+
+using namespace cunfft;
+
+// =========================================================== C ABI ======
+namespace {
+
+template <typename T> void api_copy_in(typename Traits<T>::plan *p, bool adjoint)
+{
+    using C2 = typename Traits<T>::hc;
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_host<T>(p);
+        need_device<T>(p);
+        const bool timing = p->flags & CUNFFT_MEASURE_TIME;
+        double t = 0;
+        if (adjoint)
+            t += timed_copy(im, timing, p->f_gpu, p->f, sizeof(C2) * p->M_total, hipMemcpyHostToDevice);
+        else
+            t += timed_copy(im, timing, p->fhat_gpu, p->f_hat, sizeof(C2) * p->N_total,
+                            hipMemcpyHostToDevice);
+        t += timed_copy(im, timing, p->x_gpu, p->x, sizeof(double) * p->M_total * p->d,
+                        hipMemcpyHostToDevice);
+        im->nodes_dirty = true;
+        p->CUNFFTTimes.time_COPY_IN = t;
+    });
+}
+
+template <typename T> void api_copy_out(typename Traits<T>::plan *p, bool adjoint)
+{
+    using C2 = typename Traits<T>::hc;
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_host<T>(p);
+        need_device<T>(p);
+        const bool timing = p->flags & CUNFFT_MEASURE_TIME;
+        double t;
+        if (adjoint)
+            t = timed_copy(im, timing, p->f_hat, p->fhat_gpu, sizeof(C2) * p->N_total,
+                           hipMemcpyDeviceToHost);
+        else
+            t = timed_copy(im, timing, p->f, p->f_gpu, sizeof(C2) * p->M_total, hipMemcpyDeviceToHost);
+        p->CUNFFTTimes.time_COPY_OUT = t;
+    });
+}
+
+template <typename T>
+void api_copy_dev_to(typename Traits<T>::plan *p, void *dst, const void *src, size_t bytes)
+{
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        if (!dst) throw Error(CUNFFT_ERR_INVALID_ARG, "destination is NULL");
+        need_device<T>(p);
+        timed_copy(im, false, dst, src, bytes, hipMemcpyDeviceToHost);
+    });
+}
+
+template <typename T> void api_transform(typename Traits<T>::plan *p)
+{
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        const bool timing = p->flags & CUNFFT_MEASURE_TIME;
+        double tf = 0, tc = 0;
+        run_forward<T>(p, im, reinterpret_cast<const cplx_t<T> *>(p->fhat_gpu),
+                       reinterpret_cast<cplx_t<T> *>(p->f_gpu), im->stream, timing, &tf, &tc);
+        CUNFFT_HIP(hipStreamSynchronize(im->stream));
+        p->CUNFFTTimes.time_ROC = 0.0;
+        p->CUNFFTTimes.time_FFT = tf;
+        p->CUNFFTTimes.time_CONV = tc;
+        p->CUNFFTTimes.runTime = tf + tc;
+    });
+}
+
+template <typename T> void api_adjoint(typename Traits<T>::plan *p)
+{
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        const bool timing = p->flags & CUNFFT_MEASURE_TIME;
+        double tf = 0, tc = 0;
+        run_adjoint<T>(p, im, reinterpret_cast<const cplx_t<T> *>(p->f_gpu),
+                       reinterpret_cast<cplx_t<T> *>(p->fhat_gpu), im->stream, timing, &tf, &tc);
+        CUNFFT_HIP(hipStreamSynchronize(im->stream));
+        p->CUNFFTTimes.time_ROC = 0.0;
+        p->CUNFFTTimes.time_FFT = tf;
+        p->CUNFFTTimes.time_CONV = tc;
+        p->CUNFFTTimes.runTime = tf + tc;
+    });
+}
+
+template <typename T> void api_ndft(typename Traits<T>::plan *p, bool adjoint)
+{
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        int64_t N[3];
+        for (int t = 0; t < p->d; ++t) N[t] = p->N[t];
+        if (p->N_total > 0x7fffffff) throw Error(CUNFFT_ERR_UNSUPPORTED, "NDFT needs N_total < 2^31");
+        if (adjoint)
+            launch_ndft_cols<T>(p->d, N, p->M_total, p->x_gpu,
+                                reinterpret_cast<const cplx_t<T> *>(p->f_gpu), nullptr, p->N_total,
+                                reinterpret_cast<cplx_t<T> *>(p->fhat_gpu), im->stream);
+        else
+            launch_ndft_rows<T>(p->d, N, p->M_total, p->x_gpu,
+                                reinterpret_cast<const cplx_t<T> *>(p->fhat_gpu), nullptr,
+                                p->M_total, reinterpret_cast<cplx_t<T> *>(p->f_gpu), im->stream);
+        CUNFFT_HIP(hipStreamSynchronize(im->stream));
+    });
+}
+
+template <typename T> void api_reinit(typename Traits<T>::plan *p, bool adjoint)
+{
+    using C2 = typename Traits<T>::hc;
+    guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        memset(&p->CUNFFTTimes, 0, sizeof(p->CUNFFTTimes));
+        CUNFFT_HIP(hipStreamSynchronize(im->stream));
+        init_device_para<T>(p);
+        if (p->g) memset(p->g, 0, sizeof(C2) * p->n_total);
+        if (adjoint) {
+            // src/cunfft.cu:247-257 re-allocates f_hat and g without freeing them
+            // and leaves f_hat uninitialised; here both are zeroed in place.
+            if (p->f_hat) memset(p->f_hat, 0, sizeof(C2) * p->N_total);
+        } else {
+            if (p->f) memset(p->f, 0, sizeof(C2) * p->M_total);
+        }
+    });
+}
+
+hipStream_t pick_stream(void *stream, hipStream_t own)
+{
+    return stream ? static_cast<hipStream_t>(stream) : own;
+}
+
+template <typename T>
+int api_set_nodes_dev(typename Traits<T>::plan *p, const double *x_dev, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!x_dev) throw Error(CUNFFT_ERR_INVALID_ARG, "x is NULL");
+        CUNFFT_HIP(hipMemcpyAsync(p->x_gpu, x_dev, sizeof(double) * p->M_total * p->d,
+                                  hipMemcpyDeviceToDevice, pick_stream(stream, im->stream)));
+        im->nodes_dirty = true;
+    });
+}
+
+template <typename T>
+int api_transform_dev(typename Traits<T>::plan *p, const void *fhat, void *f, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!fhat || !f) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        double a, b;
+        run_forward<T>(p, im, static_cast<const cplx_t<T> *>(fhat), static_cast<cplx_t<T> *>(f),
+                       pick_stream(stream, im->stream), false, &a, &b);
+    });
+}
+
+template <typename T>
+int api_adjoint_dev(typename Traits<T>::plan *p, const void *f, void *fhat, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!fhat || !f) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        double a, b;
+        run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(f), static_cast<cplx_t<T> *>(fhat),
+                       pick_stream(stream, im->stream), false, &a, &b);
+    });
+}
+
+template <typename T>
+int api_grad_x_dev(typename Traits<T>::plan *p, const void *fhat, const void *v, T *gx, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!fhat || !v || !gx) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        hipStream_t s = pick_stream(stream, im->stream);
+        cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
+        fft_passes<T>(p, im, fft_tables<T>(p, true), true, static_cast<const cplx_t<T> *>(fhat),
+                      nullptr, g, s);
+        launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(v), gx, s);
+    });
+}
+
+template <typename T>
+int api_ndft_dev(typename Traits<T>::plan *p, bool adjoint, const void *in, const long long *idx,
+                 long long n_idx, void *out, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!in || !out) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        if (p->N_total > 0x7fffffff) throw Error(CUNFFT_ERR_UNSUPPORTED, "NDFT needs N_total < 2^31");
+        int64_t N[3];
+        for (int t = 0; t < p->d; ++t) N[t] = p->N[t];
+        const int64_t nout = idx ? n_idx : (adjoint ? p->N_total : p->M_total);
+        hipStream_t s = pick_stream(stream, im->stream);
+        if (adjoint)
+            launch_ndft_cols<T>(p->d, N, p->M_total, p->x_gpu, static_cast<const cplx_t<T> *>(in),
+                                idx, nout, static_cast<cplx_t<T> *>(out), s);
+        else
+            launch_ndft_rows<T>(p->d, N, p->M_total, p->x_gpu, static_cast<const cplx_t<T> *>(in),
+                                idx, nout, static_cast<cplx_t<T> *>(out), s);
+    });
+}
+
+template <typename T> void api_init_plan(int dim, uint_t *n, cunfftHandle *h)
+{
+    if (!h) return;
+    *h = nullptr;
+    if (!n || dim < 1 || dim > 3) return;
+    try {
+        int64_t nn[3];
+        for (int t = 0; t < dim; ++t) {
+            if (!is_pow2(n[t]) || n[t] > CUNFFT_MAX_AXIS) return;
+            nn[t] = n[t];
+        }
+        *h = make_fft_tables(sizeof(T) == 8 ? 64 : 32, dim, nn);
+    } catch (const std::exception &e) {
+        report(e.what());
+        *h = nullptr;
+    }
+}
+
+template <typename T> void api_free_plan(typename Traits<T>::plan *p)
+{
+    if (!p) return;
+    free_fft_tables(p->fft_forward);
+    free_fft_tables(p->fft_backward);
+    p->fft_forward = p->fft_backward = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ FP64 family --
+void warmUp_createPlan(uint_t *n)
+{
+    (void)n;
+    (void)hipFree(nullptr);
+}
+void copyDataToDevice(cunfft_plan *p) { api_copy_in<double>(p, false); }
+void copyDataToDeviceAd(cunfft_plan *p) { api_copy_in<double>(p, true); }
+void copyDataToHost(cunfft_plan *p) { api_copy_out<double>(p, false); }
+void copyDataToHostAd(cunfft_plan *p) { api_copy_out<double>(p, true); }
+void copy_f_hat_ToHost(cunfft_plan *p, cunfftDoubleComplex *arr)
+{
+    if (p) api_copy_dev_to<double>(p, arr, p->fhat_gpu, sizeof(*arr) * p->N_total);
+}
+void copy_f_ToHost(cunfft_plan *p, cunfftDoubleComplex *arr)
+{
+    if (p) api_copy_dev_to<double>(p, arr, p->f_gpu, sizeof(*arr) * p->M_total);
+}
+void copy_g_ToHost(cunfft_plan *p)
+{
+    if (p) api_copy_dev_to<double>(p, p->g, p->g_gpu, sizeof(cunfftDoubleComplex) * p->n_total);
+}
+void copy_x_toHost(cunfft_plan *p, double *arr)
+{
+    if (p) api_copy_dev_to<double>(p, arr, p->x_gpu, sizeof(double) * p->M_total * p->d);
+}
+void cunfft_transform(cunfft_plan *p) { api_transform<double>(p); }
+void cunfft_adjoint(cunfft_plan *p) { api_adjoint<double>(p); }
+void cunfft_initPlan(int dim, uint_t *n, cunfftHandle *h) { api_init_plan<double>(dim, n, h); }
+void cunfft_freePlan(cunfft_plan *p) { api_free_plan<double>(p); }
+void cunfft_initDevicePara(cunfft_plan *p)
+{
+    guard(p, [&] {
+        impl_of<double>(p);
+        init_device_para<double>(p);
+    });
+}
+void cunfft_init(cunfft_plan *p, int dim, uint_t *N, uint_t M)
+{
+    api_init_guru<double>(p, dim, N, M, nullptr, CUNFFT_DEFAULT_CUT_OFF, CUFFT_INIT | CUFFT_BOTH, true);
+}
+void cunfft_initGuru(cunfft_plan *p, int dim, uint_t *N, uint_t M, uint_t *n, unsigned flags)
+{
+    api_init_guru<double>(p, dim, N, M, n, CUNFFT_DEFAULT_CUT_OFF, flags, false);
+}
+void cunfft_initGuru_m(cunfft_plan *p, int dim, uint_t *N, uint_t M, uint_t *n, int m,
+                       unsigned flags)
+{
+    api_init_guru<double>(p, dim, N, M, n, m, flags, n == nullptr);
+}
+void cunfft_initHelp(cunfft_plan *p)
+{
+    if (!p) return;
+    p->status = CUNFFT_SUCCESS;
+    p->impl = nullptr;
+    guard(p, [&] { init_help<double>(p); });
+}
+void cunfft_reinit(cunfft_plan *p) { api_reinit<double>(p, false); }
+void cunfft_reinitAd(cunfft_plan *p) { api_reinit<double>(p, true); }
+void cunfft_finalize(cunfft_plan *p)
+{
+    if (p) finalize<double>(p);
+}
+void cunfft_finalizeDevice(cunfft_plan *p)
+{
+    if (!p) return;
+    Impl<double> *im = static_cast<Impl<double> *>(p->impl);
+    if (im) (void)hipStreamSynchronize(im->stream);
+    free_device<double>(p);
+}
+void resetCUNFFTTimes(NFFTTimeSpec *t)
+{
+    if (t) memset(t, 0, sizeof(*t));
+}
+void cundft_transform(cunfft_plan *p) { api_ndft<double>(p, false); }
+void cundft_adjoint(cunfft_plan *p) { api_ndft<double>(p, true); }
+void nfft_init_fg_exp_l(double *out, const double b, const int offset)
+{
+    if (out) fg_exp_l(out, b, offset, CUNFFT_DEFAULT_CUT_OFF);
+}
+void nfft_init_fg_exp_l_m(double *out, const double b, const int offset, int m)
+{
+    if (out && m >= 1 && m <= CUNFFT_MAX_CUT_OFF) fg_exp_l(out, b, offset, m);
+}
+// src/cunfft.cu:79-90
+void setProperties(uint_t *N, uint_t *n, double *b, double *sigma, int dim, kdata *k)
+{
+    if (!N || !n || !b || !sigma || !k || dim < 1 || dim > 3) return;
+    for (int t = 0; t < dim; ++t) {
+        k->N[t] = N[t];
+        k->n[t] = n[t];
+        k->b[t] = b[t];
+        k->bsqrt[t] = sqrt(b[t]);
+        k->fac[t] = (M_PI * M_PI * b[t]) / ((double)n[t] * (double)n[t]);
+        k->sigma[t] = sigma[t];
+    }
+}
+void resetDevice(void) { (void)hipDeviceSynchronize(); }
+int cunfft_last_error(const cunfft_plan *p) { return p ? p->status : CUNFFT_ERR_INVALID_ARG; }
+const char *cunfft_error_string(int status)
+{
+    switch (status) {
+    case CUNFFT_SUCCESS: return "success";
+    case CUNFFT_ERR_INVALID_ARG: return "invalid argument";
+    case CUNFFT_ERR_UNSUPPORTED: return "unsupported parameters";
+    case CUNFFT_ERR_ALLOC: return "allocation failed";
+    case CUNFFT_ERR_HIP: return "HIP runtime error";
+    case CUNFFT_ERR_STATE: return "invalid plan state";
+    default: return "unknown status";
+    }
+}
+const char *cunfft_last_message(void) { return g_last_message.c_str(); }
+int cunfft_set_nodes_dev(cunfft_plan *p, const double *x, void *s)
+{
+    return api_set_nodes_dev<double>(p, x, s);
+}
+int cunfft_transform_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, cunfftDoubleComplex *f,
+                         void *s)
+{
+    return api_transform_dev<double>(p, fhat, f, s);
+}
+int cunfft_adjoint_dev(cunfft_plan *p, const cunfftDoubleComplex *f, cunfftDoubleComplex *fhat,
+                       void *s)
+{
+    return api_adjoint_dev<double>(p, f, fhat, s);
+}
+int cunfft_grad_x_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const cunfftDoubleComplex *v,
+                      double *gx, void *s)
+{
+    return api_grad_x_dev<double>(p, fhat, v, gx, s);
+}
+int cundft_transform_rows_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const long long *rows,
+                              long long n_rows, cunfftDoubleComplex *out, void *s)
+{
+    return api_ndft_dev<double>(p, false, fhat, rows, n_rows, out, s);
+}
+int cundft_adjoint_cols_dev(cunfft_plan *p, const cunfftDoubleComplex *f, const long long *cols,
+                            long long n_cols, cunfftDoubleComplex *out, void *s)
+{
+    return api_ndft_dev<double>(p, true, f, cols, n_cols, out, s);
+}
+
+// ------------------------------------------------------------ FP32 family --
+void cunfftf_copyDataToDevice(cunfftf_plan *p) { api_copy_in<float>(p, false); }
+void cunfftf_copyDataToDeviceAd(cunfftf_plan *p) { api_copy_in<float>(p, true); }
+void cunfftf_copyDataToHost(cunfftf_plan *p) { api_copy_out<float>(p, false); }
+void cunfftf_copyDataToHostAd(cunfftf_plan *p) { api_copy_out<float>(p, true); }
+void cunfftf_copy_f_hat_ToHost(cunfftf_plan *p, cunfftFloatComplex *arr)
+{
+    if (p) api_copy_dev_to<float>(p, arr, p->fhat_gpu, sizeof(*arr) * p->N_total);
+}
+void cunfftf_copy_f_ToHost(cunfftf_plan *p, cunfftFloatComplex *arr)
+{
+    if (p) api_copy_dev_to<float>(p, arr, p->f_gpu, sizeof(*arr) * p->M_total);
+}
+void cunfftf_copy_g_ToHost(cunfftf_plan *p)
+{
+    if (p) api_copy_dev_to<float>(p, p->g, p->g_gpu, sizeof(cunfftFloatComplex) * p->n_total);
+}
+void cunfftf_copy_x_toHost(cunfftf_plan *p, double *arr)
+{
+    if (p) api_copy_dev_to<float>(p, arr, p->x_gpu, sizeof(double) * p->M_total * p->d);
+}
+void cunfftf_transform(cunfftf_plan *p) { api_transform<float>(p); }
+void cunfftf_adjoint(cunfftf_plan *p) { api_adjoint<float>(p); }
+void cunfftf_initPlan(int dim, uint_t *n, cunfftHandle *h) { api_init_plan<float>(dim, n, h); }
+void cunfftf_freePlan(cunfftf_plan *p) { api_free_plan<float>(p); }
+void cunfftf_initDevicePara(cunfftf_plan *p)
+{
+    guard(p, [&] {
+        impl_of<float>(p);
+        init_device_para<float>(p);
+    });
+}
+void cunfftf_init(cunfftf_plan *p, int dim, uint_t *N, uint_t M)
+{
+    api_init_guru<float>(p, dim, N, M, nullptr, CUNFFT_DEFAULT_CUT_OFF, CUFFT_INIT | CUFFT_BOTH, true);
+}
+void cunfftf_initGuru(cunfftf_plan *p, int dim, uint_t *N, uint_t M, uint_t *n, unsigned flags)
+{
+    api_init_guru<float>(p, dim, N, M, n, CUNFFT_DEFAULT_CUT_OFF, flags, false);
+}
+void cunfftf_initGuru_m(cunfftf_plan *p, int dim, uint_t *N, uint_t M, uint_t *n, int m,
+                        unsigned flags)
+{
+    api_init_guru<float>(p, dim, N, M, n, m, flags, n == nullptr);
+}
+void cunfftf_initHelp(cunfftf_plan *p)
+{
+    if (!p) return;
+    p->status = CUNFFT_SUCCESS;
+    p->impl = nullptr;
+    guard(p, [&] { init_help<float>(p); });
+}
+void cunfftf_reinit(cunfftf_plan *p) { api_reinit<float>(p, false); }
+void cunfftf_reinitAd(cunfftf_plan *p) { api_reinit<float>(p, true); }
+void cunfftf_finalize(cunfftf_plan *p)
+{
+    if (p) finalize<float>(p);
+}
+void cunfftf_finalizeDevice(cunfftf_plan *p)
+{
+    if (!p) return;
+    Impl<float> *im = static_cast<Impl<float> *>(p->impl);
+    if (im) (void)hipStreamSynchronize(im->stream);
+    free_device<float>(p);
+}
+void cunfftf_ndft_transform(cunfftf_plan *p) { api_ndft<float>(p, false); }
+void cunfftf_ndft_adjoint(cunfftf_plan *p) { api_ndft<float>(p, true); }
+int cunfftf_last_error(const cunfftf_plan *p) { return p ? p->status : CUNFFT_ERR_INVALID_ARG; }
+int cunfftf_set_nodes_dev(cunfftf_plan *p, const double *x, void *s)
+{
+    return api_set_nodes_dev<float>(p, x, s);
+}
+int cunfftf_transform_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat, cunfftFloatComplex *f,
+                          void *s)
+{
+    return api_transform_dev<float>(p, fhat, f, s);
+}
+int cunfftf_adjoint_dev(cunfftf_plan *p, const cunfftFloatComplex *f, cunfftFloatComplex *fhat,
+                        void *s)
+{
+    return api_adjoint_dev<float>(p, f, fhat, s);
+}
+int cunfftf_grad_x_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat, const cunfftFloatComplex *v,
+                       float *gx, void *s)
+{
+    return api_grad_x_dev<float>(p, fhat, v, gx, s);
+}
+int cunfftf_ndft_transform_rows_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat,
+                                    const long long *rows, long long n_rows, cunfftFloatComplex *out,
+                                    void *s)
+{
+    return api_ndft_dev<float>(p, false, fhat, rows, n_rows, out, s);
+}
+int cunfftf_ndft_adjoint_cols_dev(cunfftf_plan *p, const cunfftFloatComplex *f, const long long *cols,
+                                  long long n_cols, cunfftFloatComplex *out, void *s)
+{
+    return api_ndft_dev<float>(p, true, f, cols, n_cols, out, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+/*
+ * cunfft.h -- C ABI of the MI355X-native NFFT library (libcunfft_amd.so).
+ *
+ * Drop-in for the plan surface of CUNFFT 1.0, src/cunfft.h:35-261 and
+ * src/cunfft_typedefs.h:36-178: same function names, argument order, void
+ * returns and plan field names.  Two precision families are exported:
+ *
+ *   FP64  cunfft_plan,  cunfft_*, copyDataTo*, cundft_*   (reference names;
+ *         the reference's default build, CMakeLists.txt:173)
+ *   FP32  cunfftf_plan, cunfftf_*                          (suffixed family)
+ *
+ * A caller that defines CUNFFT_SINGLE_PRECISION before including this header
+ * gets the reference names mapped onto the FP32 family, the way the
+ * reference's CUNFFT_DOUBLE_PRECISION switch selects gpuComplex/dTyp
+ * (src/cunfft_typedefs.h:59-69).  Node coordinates x are double in both
+ * families, so the node-to-grid index split is always done in FP64.
+ *
+ * Differences from the reference, all deliberate:
+ *  - m (cut-off) is a runtime plan parameter: cunfft_init uses
+ *    CUNFFT_DEFAULT_CUT_OFF (6, as CMakeLists.txt:170), cunfft_initGuru_m
+ *    takes it explicitly.
+ *  - errors never exit(): every call records a status in plan->status,
+ *    readable with cunfft_last_error(); after an error every call except
+ *    finalize is a no-op until the plan is finalized.
+ *  - the FFT handles are opaque (cunfftHandle) and the device state lives in
+ *    the plan (plan->impl), so several plans may live at once; one plan is
+ *    not re-entrant.
+ *  - finalize does not reset the device.
+ */
+#ifndef CUNFFT_AMD_H_
+#define CUNFFT_AMD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ constants -- */
+
+#define CUNFFT_DEFAULT_CUT_OFF 6
+#define CUNFFT_MAX_CUT_OFF 16
+#define CUNFFT_MAX_DIM 3
+/** Largest oversampled axis length n_t handled by one LDS-resident pass. */
+#define CUNFFT_MAX_AXIS 4096
+
+/* plan flags, same bits as src/cunfft_typedefs.h:46-53 */
+#define CUFFT_INIT  (1U << 0)
+#define CUFFT_BOTH  (1U << 1)
+#define CUFFT_FORW  (1U << 2)
+#define CUFFT_BACKW (1U << 3)
+/* additions */
+/** Record per-step device times in plan->CUNFFTTimes (MEASURED_TIMES at
+ *  run time instead of compile time, src/timeMeasurement.h:38-48). */
+#define CUNFFT_MEASURE_TIME   (1U << 8)
+/** Do not allocate the pinned host arrays x, f_hat, f, g (device-pointer
+ *  callers only; the copy functions then fail with CUNFFT_ERR_STATE). */
+#define CUNFFT_NO_HOST_BUFFERS (1U << 9)
+/** Adjoint gridding with one global atomic per tap instead of the binned
+ *  tile scatter (the plain path). */
+#define CUNFFT_ATOMIC_SCATTER (1U << 10)
+/** Forward gridding over unsorted nodes (one node per lane, plain path). */
+#define CUNFFT_PLAIN_GATHER   (1U << 11)
+
+/* status codes */
+#define CUNFFT_SUCCESS          0
+#define CUNFFT_ERR_INVALID_ARG  1
+#define CUNFFT_ERR_UNSUPPORTED  2
+#define CUNFFT_ERR_ALLOC        3
+#define CUNFFT_ERR_HIP          4
+#define CUNFFT_ERR_STATE        5
+
+/* ---------------------------------------------------------------- types -- */
+
+/** Index type: the reference's LARGE_INPUT build (CMakeLists.txt:172),
+ *  src/cunfft_typedefs.h:71-87. */
+typedef ptrdiff_t uint_t;
+typedef ptrdiff_t sint_t;
+
+/** Interleaved complex with .x = re, .y = im (as cuDoubleComplex). */
+typedef struct { double x, y; } cunfftDoubleComplex;
+typedef struct { float x, y; } cunfftFloatComplex;
+
+/** Opaque FFT plan handle (replaces cufftHandle). */
+typedef void *cunfftHandle;
+
+/** Per-step times in seconds, src/timeMeasurement.h:22-36. time_ROC stays
+ *  0 because roll-off, pad and crop are fused into the FFT passes and are
+ *  counted in time_FFT. */
+typedef struct {
+    double runTime;
+    double time_ROC;
+    double time_CONV;
+    double time_FFT;
+    double time_COPY_IN;
+    double time_COPY_OUT;
+    double time_else;
+} NFFTTimeSpec;
+
+/** Per-axis transform constants, src/cunfft_typedefs.h:98-112. */
+typedef struct {
+    uint_t N[CUNFFT_MAX_DIM];
+    uint_t n[CUNFFT_MAX_DIM];
+    double b[CUNFFT_MAX_DIM];
+    double bsqrt[CUNFFT_MAX_DIM];
+    double fac[CUNFFT_MAX_DIM];
+    double sigma[CUNFFT_MAX_DIM];
+} kdata;
+
+/* Plan fields of src/cunfft_typedefs.h:118-178 (same names, same meaning),
+ * followed by the opaque device state and the status word. */
+#define CUNFFT_PLAN_FIELDS(CT)                                                 \
+    uint_t N_total;                                                            \
+    uint_t M_total;                                                            \
+    int d;                                                                     \
+    uint_t *N;                                                                 \
+    double *x;          /* pinned host nodes, x[d*j+t] in [-0.5,0.5) */        \
+    CT *f_hat;          /* pinned host coefficients, N_total */                \
+    CT *f;              /* pinned host samples, M_total */                     \
+    double *sigma;                                                             \
+    uint_t *n;                                                                 \
+    uint_t n_total;                                                            \
+    int m;                                                                     \
+    double *b;                                                                 \
+    CT *g;              /* pinned host grid, n_total (copy_g_ToHost) */        \
+    CT *g_hat;          /* unused, kept for layout parity */                   \
+    double *fg_exp_l_cpu; /* exp(-l^2/b_t), (2m+2) per axis */                 \
+    CT *g_gpu;                                                                 \
+    CT *fhat_gpu;                                                              \
+    double *x_gpu;                                                             \
+    CT *f_gpu;                                                                 \
+    cunfftHandle fft_forward;                                                  \
+    cunfftHandle fft_backward;                                                 \
+    NFFTTimeSpec CUNFFTTimes;                                                  \
+    unsigned flags;                                                            \
+    void *impl;                                                                \
+    int status;
+
+typedef struct { CUNFFT_PLAN_FIELDS(cunfftDoubleComplex) } cunfft_plan;
+typedef struct { CUNFFT_PLAN_FIELDS(cunfftFloatComplex) } cunfftf_plan;
+
+/* ======================================================= FP64 family ==== */
+/* Each declaration cites the reference declaration it replaces. */
+
+/* src/cunfft.h:35 */
+void warmUp_createPlan(uint_t *n);
+/* src/cunfft.h:47 */
+void copyDataToDevice(cunfft_plan *plan);
+/* src/cunfft.h:57 */
+void copyDataToDeviceAd(cunfft_plan *plan);
+/* src/cunfft.h:66 */
+void copyDataToHost(cunfft_plan *plan);
+/* src/cunfft.h:75 */
+void copyDataToHostAd(cunfft_plan *plan);
+/* src/cunfft.h:83 */
+void copy_f_hat_ToHost(cunfft_plan *plan, cunfftDoubleComplex *arr);
+/* src/cunfft.h:91 */
+void copy_f_ToHost(cunfft_plan *plan, cunfftDoubleComplex *arr);
+/* src/cunfft.h:98 */
+void copy_g_ToHost(cunfft_plan *plan);
+/* src/cunfft.h:106 */
+void copy_x_toHost(cunfft_plan *plan, double *arr);
+/* src/cunfft.h:117 */
+void cunfft_transform(cunfft_plan *plan);
+/* src/cunfft.h:124 */
+void cunfft_adjoint(cunfft_plan *plan);
+/* src/cunfft.h:133 (cufftHandle -> cunfftHandle) */
+void cunfft_initPlan(int dim, uint_t *n, cunfftHandle *h);
+/* src/cunfft.h:139 */
+void cunfft_freePlan(cunfft_plan *p);
+/* src/cunfft.h:145 */
+void cunfft_initDevicePara(cunfft_plan *plan);
+/* src/cunfft.h:156 */
+void cunfft_init(cunfft_plan *plan, int dim, uint_t *N, uint_t M_total);
+/* src/cunfft.h:168 */
+void cunfft_initGuru(cunfft_plan *plan, int dim, uint_t *N, uint_t M_total,
+                     uint_t *n, unsigned flags);
+/* src/cunfft.h:177 */
+void cunfft_initHelp(cunfft_plan *plan);
+/* src/cunfft.h:185 */
+void cunfft_reinit(cunfft_plan *plan);
+/* src/cunfft.h:194 */
+void cunfft_reinitAd(cunfft_plan *plan);
+/* src/cunfft.h:200 */
+void cunfft_finalize(cunfft_plan *plan);
+/* src/cunfft.h:206 */
+void cunfft_finalizeDevice(cunfft_plan *plan);
+/* src/cunfft.h:212 */
+void resetCUNFFTTimes(NFFTTimeSpec *times);
+/* src/cunfft.h:222 */
+void cundft_transform(cunfft_plan *plan);
+/* src/cunfft.h:228 */
+void cundft_adjoint(cunfft_plan *plan);
+/* src/cunfft.h:242; FILTER_SIZE = 2*CUNFFT_DEFAULT_CUT_OFF+2 entries */
+void nfft_init_fg_exp_l(double *fg_exp_l, const double b, const int offset);
+/* src/cunfft.h:256 */
+void setProperties(uint_t *N, uint_t *n, double *b, double *sigma, int dim,
+                   kdata *k_cpu);
+/* src/cunfft.h:261: synchronizes the device and releases this library's
+ * cached device state; the primary context is left alive. */
+void resetDevice(void);
+
+/* ---- additions (no counterpart in the reference) ---- */
+
+/* cunfft_initGuru with a runtime cut-off m (the reference fixes
+ * m = CUT_OFF at compile time, src/cunfft.cu:294,322). */
+void cunfft_initGuru_m(cunfft_plan *plan, int dim, uint_t *N, uint_t M_total,
+                       uint_t *n, int m, unsigned flags);
+/* Status of the last call on this plan (CUNFFT_SUCCESS or CUNFFT_ERR_*). */
+int cunfft_last_error(const cunfft_plan *plan);
+/* FP32 variant of nfft_init_fg_exp_l with explicit cut-off. */
+void nfft_init_fg_exp_l_m(double *fg_exp_l, const double b, const int offset, int m);
+/* Text for a status code. */
+const char *cunfft_error_string(int status);
+/* Message of the last error recorded by this thread (empty if none). */
+const char *cunfft_last_message(void);
+
+/* Device-pointer entry points: operate on caller-owned device buffers on the
+ * caller's HIP stream (stream may be NULL for the plan's own stream).  They
+ * allocate nothing and return the status.  Buffers: fhat N_total complex,
+ * f M_total complex, x d*M_total doubles. */
+/* Replace the plan's nodes (device copy) and rebuild the node binning. */
+int cunfft_set_nodes_dev(cunfft_plan *plan, const double *x_dev, void *stream);
+/* f = A fhat (forward NFFT, src/cunfft.cu:196-207). */
+int cunfft_transform_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
+                         cunfftDoubleComplex *f_dev, void *stream);
+/* fhat = A^H f (adjoint NFFT, src/cunfft.cu:209-224). */
+int cunfft_adjoint_dev(cunfft_plan *plan, const cunfftDoubleComplex *f_dev,
+                       cunfftDoubleComplex *fhat_dev, void *stream);
+/* Backward of f = A(x) fhat with respect to x: gx[j*d+t] =
+ * Re(conj(v_j) * d f_j / d x_jt) for upstream v, the derivative-of-Gaussian
+ * gather over the forward grid of fhat (weights recomputed, not stored). */
+int cunfft_grad_x_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
+                      const cunfftDoubleComplex *v_dev, double *gx_dev, void *stream);
+/* Direct NDFT on the GPU in FP64 for the listed outputs only (rows of f or
+ * entries of fhat by flat index); rows == NULL means all outputs. */
+int cundft_transform_rows_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
+                              const long long *rows_dev, long long n_rows,
+                              cunfftDoubleComplex *out_dev, void *stream);
+int cundft_adjoint_cols_dev(cunfft_plan *plan, const cunfftDoubleComplex *f_dev,
+                            const long long *cols_dev, long long n_cols,
+                            cunfftDoubleComplex *out_dev, void *stream);
+
+/* ======================================================= FP32 family ==== */
+
+void cunfftf_copyDataToDevice(cunfftf_plan *plan);
+void cunfftf_copyDataToDeviceAd(cunfftf_plan *plan);
+void cunfftf_copyDataToHost(cunfftf_plan *plan);
+void cunfftf_copyDataToHostAd(cunfftf_plan *plan);
+void cunfftf_copy_f_hat_ToHost(cunfftf_plan *plan, cunfftFloatComplex *arr);
+void cunfftf_copy_f_ToHost(cunfftf_plan *plan, cunfftFloatComplex *arr);
+void cunfftf_copy_g_ToHost(cunfftf_plan *plan);
+void cunfftf_copy_x_toHost(cunfftf_plan *plan, double *arr);
+void cunfftf_transform(cunfftf_plan *plan);
+void cunfftf_adjoint(cunfftf_plan *plan);
+void cunfftf_initPlan(int dim, uint_t *n, cunfftHandle *h);
+void cunfftf_freePlan(cunfftf_plan *p);
+void cunfftf_initDevicePara(cunfftf_plan *plan);
+void cunfftf_init(cunfftf_plan *plan, int dim, uint_t *N, uint_t M_total);
+void cunfftf_initGuru(cunfftf_plan *plan, int dim, uint_t *N, uint_t M_total,
+                      uint_t *n, unsigned flags);
+void cunfftf_initHelp(cunfftf_plan *plan);
+void cunfftf_reinit(cunfftf_plan *plan);
+void cunfftf_reinitAd(cunfftf_plan *plan);
+void cunfftf_finalize(cunfftf_plan *plan);
+void cunfftf_finalizeDevice(cunfftf_plan *plan);
+void cunfftf_ndft_transform(cunfftf_plan *plan);
+void cunfftf_ndft_adjoint(cunfftf_plan *plan);
+void cunfftf_initGuru_m(cunfftf_plan *plan, int dim, uint_t *N, uint_t M_total,
+                        uint_t *n, int m, unsigned flags);
+int cunfftf_last_error(const cunfftf_plan *plan);
+int cunfftf_set_nodes_dev(cunfftf_plan *plan, const double *x_dev, void *stream);
+int cunfftf_transform_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
+                          cunfftFloatComplex *f_dev, void *stream);
+int cunfftf_adjoint_dev(cunfftf_plan *plan, const cunfftFloatComplex *f_dev,
+                        cunfftFloatComplex *fhat_dev, void *stream);
+int cunfftf_grad_x_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
+                       const cunfftFloatComplex *v_dev, float *gx_dev, void *stream);
+int cunfftf_ndft_transform_rows_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
+                                    const long long *rows_dev, long long n_rows,
+                                    cunfftFloatComplex *out_dev, void *stream);
+int cunfftf_ndft_adjoint_cols_dev(cunfftf_plan *plan, const cunfftFloatComplex *f_dev,
+                                  const long long *cols_dev, long long n_cols,
+                                  cunfftFloatComplex *out_dev, void *stream);
+
+/* --------------------------------------------- reference-name mapping ---- */
+
+#ifdef CUNFFT_SINGLE_PRECISION
+typedef cunfftFloatComplex gpuComplex;
+typedef float dTyp;
+#define cunfft_plan cunfftf_plan
+#define copyDataToDevice cunfftf_copyDataToDevice
+#define copyDataToDeviceAd cunfftf_copyDataToDeviceAd
+#define copyDataToHost cunfftf_copyDataToHost
+#define copyDataToHostAd cunfftf_copyDataToHostAd
+#define copy_f_hat_ToHost cunfftf_copy_f_hat_ToHost
+#define copy_f_ToHost cunfftf_copy_f_ToHost
+#define copy_g_ToHost cunfftf_copy_g_ToHost
+#define copy_x_toHost cunfftf_copy_x_toHost
+#define cunfft_transform cunfftf_transform
+#define cunfft_adjoint cunfftf_adjoint
+#define cunfft_initPlan cunfftf_initPlan
+#define cunfft_freePlan cunfftf_freePlan
+#define cunfft_initDevicePara cunfftf_initDevicePara
+#define cunfft_init cunfftf_init
+#define cunfft_initGuru cunfftf_initGuru
+#define cunfft_initHelp cunfftf_initHelp
+#define cunfft_reinit cunfftf_reinit
+#define cunfft_reinitAd cunfftf_reinitAd
+#define cunfft_finalize cunfftf_finalize
+#define cunfft_finalizeDevice cunfftf_finalizeDevice
+#define cundft_transform cunfftf_ndft_transform
+#define cundft_adjoint cunfftf_ndft_adjoint
+#define cunfft_initGuru_m cunfftf_initGuru_m
+#define cunfft_last_error cunfftf_last_error
+#define cunfft_set_nodes_dev cunfftf_set_nodes_dev
+#define cunfft_transform_dev cunfftf_transform_dev
+#define cunfft_adjoint_dev cunfftf_adjoint_dev
+#define cunfft_grad_x_dev cunfftf_grad_x_dev
+#define cundft_transform_rows_dev cunfftf_ndft_transform_rows_dev
+#define cundft_adjoint_cols_dev cunfftf_ndft_adjoint_cols_dev
+#else
+typedef cunfftDoubleComplex gpuComplex;
+typedef double dTyp;
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUNFFT_AMD_H_ */

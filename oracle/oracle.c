@@ -1,0 +1,558 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the CUNFFT algorithm (test checker).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker, never by the product path.
+ * Each function names the reference file:line it restates; the header lists
+ * the conventions.  Written from the mathematics of those lines, not copied.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+#define MAXD 3
+
+/* ---------------------------------------------------------------- inputs -- */
+
+/* glibc drand48: X <- (0x5DEECE66D X + 0xB) mod 2^48, value = X / 2^48. */
+double orc_drand48(uint64_t *state)
+{
+    const uint64_t a = 0x5DEECE66DULL, c = 0xBULL, mask = (1ULL << 48) - 1ULL;
+    *state = (a * (*state) + c) & mask;
+    return ldexp((double)(*state), -48);
+}
+
+/* src/cuda_util.cpp:151-160 (getExampleData_uniDistr) with the coefficient
+ * generator of src/cuda_util.cpp:204-212. */
+void orc_example_data_unidistr(uint64_t *state, int64_t N_total, int64_t M, int d,
+                               double *fhat, double *x)
+{
+    for (int64_t k = 0; k < N_total; ++k) {
+        fhat[2 * k] = orc_drand48(state);
+        fhat[2 * k + 1] = orc_drand48(state);
+    }
+    for (int64_t i = 0; i < M * (int64_t)d; ++i)
+        x[i] = orc_drand48(state) - 0.5;
+}
+
+/* src/cuda_util.cpp:162-171 (getExampleDataAd_uniDistr). */
+void orc_example_data_ad_unidistr(uint64_t *state, int64_t M, int d, double *f,
+                                  double *x)
+{
+    for (int64_t j = 0; j < M; ++j) {
+        f[2 * j] = orc_drand48(state);
+        f[2 * j + 1] = orc_drand48(state);
+    }
+    for (int64_t i = 0; i < M * (int64_t)d; ++i)
+        x[i] = orc_drand48(state) - 0.5;
+}
+
+/* ------------------------------------------------------- window, indices -- */
+
+/* src/cunfft.cu:342-350: sigma = n/N, b = 2 sigma/(2 sigma-1) * m/pi. */
+double orc_shape_b(int64_t N, int64_t n, int m)
+{
+    double sigma = (double)n / (double)N;
+    return (2.0 * sigma) / (2.0 * sigma - 1.0) * ((double)m / M_PI);
+}
+
+/* src/cunfft_kernel.cu:175-182: llrint rounds half to even in the default
+ * rounding mode; signbit(-0.0) is 1. */
+int64_t orc_window_start(int64_t n, double x, int m)
+{
+    return (int64_t)llrint((double)n * x) - (signbit(x) ? 1 : 0) - m;
+}
+
+/* src/cunfft_kernel.cu:162-166: exp(-t^2 b' pi) sqrt(b') with b' = 1/(pi b). */
+double orc_phi(double t, double b)
+{
+    return exp(-(t * t) / b) / sqrt(M_PI * b);
+}
+
+static int64_t pmod(int64_t a, int64_t n)
+{
+    int64_t r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+static int64_t prod(const int64_t *v, int d)
+{
+    int64_t p = 1;
+    for (int t = 0; t < d; ++t) p *= v[t];
+    return p;
+}
+
+/* ------------------------------------------------------------- roll-off -- */
+
+/* Forward D: src/cunfft_wrapper.cu:32-75 (memset, then kernelRollOf_{1,2,3}d,
+ * src/cunfft_kernel.cu:240-254, 286-312, 355-412): every centred k lands at
+ * (k mod n) multiplied by exp(sum_t fac_t k_t^2); the rest of the grid is 0. */
+void orc_rolloff_pad(int d, const int64_t *N, const int64_t *n, int m,
+                     const double *fhat, double *ghat)
+{
+    int64_t Nt = prod(N, d), nt = prod(n, d);
+    double fac[MAXD];
+    for (int t = 0; t < d; ++t) {
+        double b = orc_shape_b(N[t], n[t], m);
+        fac[t] = M_PI * M_PI * b / ((double)n[t] * (double)n[t]);
+    }
+    memset(ghat, 0, sizeof(double) * 2 * (size_t)nt);
+    for (int64_t kl = 0; kl < Nt; ++kl) {
+        int64_t rem = kl, gidx = 0, gstride = 1;
+        double e = 0.0;
+        int64_t k[MAXD];
+        for (int t = d - 1; t >= 0; --t) {
+            k[t] = rem % N[t] - N[t] / 2;
+            rem /= N[t];
+        }
+        for (int t = d - 1; t >= 0; --t) {
+            gidx += pmod(k[t], n[t]) * gstride;
+            gstride *= n[t];
+            e += fac[t] * (double)k[t] * (double)k[t];
+        }
+        double s = exp(e);
+        ghat[2 * gidx] = fhat[2 * kl] * s;
+        ghat[2 * gidx + 1] = fhat[2 * kl + 1] * s;
+    }
+}
+
+/* Adjoint D^T: src/cunfft_wrapper.cu:77-121 and kernelRollOf_adjoint_{1,2,3}d
+ * (src/cunfft_kernel.cu:264-275, 318-343, 418-467). */
+void orc_crop_rolloff(int d, const int64_t *N, const int64_t *n, int m,
+                      const double *g, double *fhat)
+{
+    int64_t Nt = prod(N, d);
+    double fac[MAXD];
+    for (int t = 0; t < d; ++t) {
+        double b = orc_shape_b(N[t], n[t], m);
+        fac[t] = M_PI * M_PI * b / ((double)n[t] * (double)n[t]);
+    }
+    for (int64_t kl = 0; kl < Nt; ++kl) {
+        int64_t rem = kl, gidx = 0, gstride = 1;
+        double e = 0.0;
+        int64_t k[MAXD];
+        for (int t = d - 1; t >= 0; --t) {
+            k[t] = rem % N[t] - N[t] / 2;
+            rem /= N[t];
+        }
+        for (int t = d - 1; t >= 0; --t) {
+            gidx += pmod(k[t], n[t]) * gstride;
+            gstride *= n[t];
+            e += fac[t] * (double)k[t] * (double)k[t];
+        }
+        double s = exp(e);
+        fhat[2 * kl] = g[2 * gidx] * s;
+        fhat[2 * kl + 1] = g[2 * gidx + 1] * s;
+    }
+}
+
+/* ------------------------------------------------------------------ FFT -- */
+
+/* Iterative radix-2 FFT of one contiguous line (length L, power of two),
+ * unnormalised, kernel e^{sign 2 pi i j k / L}; twiddles from sin/cos of the
+ * exact angle, one call per twiddle. */
+static void fft_line(double *a, int64_t L, int sign, const double *tw)
+{
+    for (int64_t i = 1, j = 0; i < L; ++i) {
+        int64_t bit = L >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double tr = a[2 * i], ti = a[2 * i + 1];
+            a[2 * i] = a[2 * j]; a[2 * i + 1] = a[2 * j + 1];
+            a[2 * j] = tr; a[2 * j + 1] = ti;
+        }
+    }
+    for (int64_t len = 2; len <= L; len <<= 1) {
+        int64_t half = len >> 1, step = L / len;
+        for (int64_t s = 0; s < L; s += len) {
+            for (int64_t q = 0; q < half; ++q) {
+                double wr = tw[2 * (q * step)], wi = sign * tw[2 * (q * step) + 1];
+                double *p0 = a + 2 * (s + q), *p1 = a + 2 * (s + q + half);
+                double xr = p1[0] * wr - p1[1] * wi, xi = p1[0] * wi + p1[1] * wr;
+                p1[0] = p0[0] - xr; p1[1] = p0[1] - xi;
+                p0[0] += xr; p0[1] += xi;
+            }
+        }
+    }
+}
+
+/* cufftExecZ2Z(..., CUFFT_FORWARD / CUFFT_INVERSE), src/cunfft_wrapper.cu:288-329. */
+void orc_fft(int d, const int64_t *n, double *g, int sign)
+{
+    int64_t nt = prod(n, d);
+    for (int t = 0; t < d; ++t) {
+        int64_t L = n[t], stride = 1;
+        for (int s = t + 1; s < d; ++s) stride *= n[s];
+        int64_t lines = nt / L;
+        double *tw = (double *)malloc(sizeof(double) * 2 * (size_t)L);
+        for (int64_t q = 0; q < L; ++q) {
+            double ang = 2.0 * M_PI * (double)q / (double)L;
+            tw[2 * q] = cos(ang);
+            tw[2 * q + 1] = sin(ang);
+        }
+        /* sign convention: forward uses e^{-...}; fft_line uses e^{sign ...} */
+#pragma omp parallel
+        {
+            double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)L);
+#pragma omp for schedule(static)
+            for (int64_t li = 0; li < lines; ++li) {
+                int64_t outer = li / stride, inner = li % stride;
+                int64_t base = outer * L * stride + inner;
+                for (int64_t q = 0; q < L; ++q) {
+                    buf[2 * q] = g[2 * (base + q * stride)];
+                    buf[2 * q + 1] = g[2 * (base + q * stride) + 1];
+                }
+                fft_line(buf, L, sign, tw);
+                for (int64_t q = 0; q < L; ++q) {
+                    g[2 * (base + q * stride)] = buf[2 * q];
+                    g[2 * (base + q * stride) + 1] = buf[2 * q + 1];
+                }
+            }
+            free(buf);
+        }
+        free(tw);
+    }
+}
+
+/* --------------------------------------------------------------- gridding -- */
+
+struct node_win {
+    int64_t idx[MAXD][64];
+    double w[MAXD][64];
+    double dw[MAXD][64];
+};
+
+static void node_window(int d, const int64_t *N, const int64_t *n, int m,
+                        const double *xj, struct node_win *nw, int want_dw)
+{
+    int W = 2 * m + 2;
+    for (int t = 0; t < d; ++t) {
+        double b = orc_shape_b(N[t], n[t], m);
+        int64_t u = orc_window_start(n[t], xj[t], m);
+        double nx = (double)n[t] * xj[t];
+        for (int l = 0; l < W; ++l) {
+            double s = nx - (double)(u + l);
+            double p = orc_phi(s, b);
+            nw->idx[t][l] = pmod(u + l, n[t]);
+            nw->w[t][l] = p;
+            /* d/dx phi(n x - l) = n * (-2 s / b) phi(s) */
+            if (want_dw) nw->dw[t][l] = (double)n[t] * (-2.0 * s / b) * p;
+        }
+    }
+}
+
+/* Forward B: src/cunfft_kernel.cu:514-1093 (kernelConvWithGauss_{1,2,3}d);
+ * the FG_PSI recurrence of :491-505 is an algebraic rewrite of the same
+ * window values, evaluated here directly. */
+void orc_gather(int d, const int64_t *N, const int64_t *n, int m, const double *g,
+                int64_t M, const double *x, double *f)
+{
+    int W = 2 * m + 2;
+    int64_t s1 = d == 3 ? n[2] : (d == 2 ? n[1] : 1);
+    int64_t s0 = d == 3 ? n[1] * n[2] : s1;
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < M; ++j) {
+        struct node_win nw;
+        node_window(d, N, n, m, x + d * j, &nw, 0);
+        double sr = 0.0, si = 0.0;
+        if (d == 1) {
+            for (int a = 0; a < W; ++a) {
+                const double *p = g + 2 * nw.idx[0][a];
+                sr += p[0] * nw.w[0][a];
+                si += p[1] * nw.w[0][a];
+            }
+        } else if (d == 2) {
+            for (int a = 0; a < W; ++a) {
+                double r1 = 0, i1 = 0;
+                for (int c = 0; c < W; ++c) {
+                    const double *p = g + 2 * (nw.idx[0][a] * s1 + nw.idx[1][c]);
+                    r1 += p[0] * nw.w[1][c];
+                    i1 += p[1] * nw.w[1][c];
+                }
+                sr += r1 * nw.w[0][a];
+                si += i1 * nw.w[0][a];
+            }
+        } else {
+            for (int a = 0; a < W; ++a) {
+                double r1 = 0, i1 = 0;
+                for (int c = 0; c < W; ++c) {
+                    double r2 = 0, i2 = 0;
+                    for (int e = 0; e < W; ++e) {
+                        const double *p =
+                            g + 2 * (nw.idx[0][a] * s0 + nw.idx[1][c] * s1 + nw.idx[2][e]);
+                        r2 += p[0] * nw.w[2][e];
+                        i2 += p[1] * nw.w[2][e];
+                    }
+                    r1 += r2 * nw.w[1][c];
+                    i1 += i2 * nw.w[1][c];
+                }
+                sr += r1 * nw.w[0][a];
+                si += i1 * nw.w[0][a];
+            }
+        }
+        f[2 * j] = sr;
+        f[2 * j + 1] = si;
+    }
+}
+
+/* Adjoint B^T: src/cunfft_wrapper.cu:178-247 (memset g, then
+ * kernelConvWithGauss_adjoint_{1,2,3}d, src/cunfft_kernel.cu:608-1197).
+ * Serial over nodes so the summation order is fixed. */
+void orc_scatter(int d, const int64_t *N, const int64_t *n, int m, int64_t M,
+                 const double *x, const double *f, double *g)
+{
+    int W = 2 * m + 2;
+    int64_t nt = prod(n, d);
+    int64_t s1 = d == 3 ? n[2] : (d == 2 ? n[1] : 1);
+    int64_t s0 = d == 3 ? n[1] * n[2] : s1;
+    memset(g, 0, sizeof(double) * 2 * (size_t)nt);
+    struct node_win nw;
+    for (int64_t j = 0; j < M; ++j) {
+        node_window(d, N, n, m, x + d * j, &nw, 0);
+        double fr = f[2 * j], fi = f[2 * j + 1];
+        if (d == 1) {
+            for (int a = 0; a < W; ++a) {
+                double *p = g + 2 * nw.idx[0][a];
+                p[0] += fr * nw.w[0][a];
+                p[1] += fi * nw.w[0][a];
+            }
+        } else if (d == 2) {
+            for (int a = 0; a < W; ++a)
+                for (int c = 0; c < W; ++c) {
+                    double w = nw.w[0][a] * nw.w[1][c];
+                    double *p = g + 2 * (nw.idx[0][a] * s1 + nw.idx[1][c]);
+                    p[0] += fr * w;
+                    p[1] += fi * w;
+                }
+        } else {
+            for (int a = 0; a < W; ++a)
+                for (int c = 0; c < W; ++c) {
+                    double w01 = nw.w[0][a] * nw.w[1][c];
+                    for (int e = 0; e < W; ++e) {
+                        double w = w01 * nw.w[2][e];
+                        double *p =
+                            g + 2 * (nw.idx[0][a] * s0 + nw.idx[1][c] * s1 + nw.idx[2][e]);
+                        p[0] += fr * w;
+                        p[1] += fi * w;
+                    }
+                }
+        }
+    }
+}
+
+/* cunfft_transform: ROC -> FFT(-) -> gather, src/cunfft.cu:196-207. */
+void orc_nfft_trafo(int d, const int64_t *N, const int64_t *n, int m, int64_t M,
+                    const double *x, const double *fhat, double *f, double *g_work)
+{
+    orc_rolloff_pad(d, N, n, m, fhat, g_work);
+    orc_fft(d, n, g_work, -1);
+    orc_gather(d, N, n, m, g_work, M, x, f);
+}
+
+/* cunfft_adjoint: scatter -> FFT(+) -> crop/ROC, src/cunfft.cu:209-224. */
+void orc_nfft_adjoint(int d, const int64_t *N, const int64_t *n, int m, int64_t M,
+                      const double *x, const double *f, double *fhat, double *g_work)
+{
+    orc_scatter(d, N, n, m, M, x, f, g_work);
+    orc_fft(d, n, g_work, +1);
+    orc_crop_rolloff(d, N, n, m, g_work, fhat);
+}
+
+/* ----------------------------------------------------------------- NDFT -- */
+
+static void set_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    else omp_set_num_threads(omp_get_num_procs());
+#else
+    (void)nthreads;
+#endif
+}
+
+static void k_of(int d, const int64_t *N, int64_t kl, int64_t *k)
+{
+    for (int t = d - 1; t >= 0; --t) {
+        k[t] = kl % N[t] - N[t] / 2;
+        kl /= N[t];
+    }
+}
+
+/* kernelNDFT_{1,2,3}d / NDFT_forElem, src/cunfft_kernel.cu:1207-1285;
+ * NFFT3 trafo_direct as wrapped by Tests/Scripts/ndft.cpp:11-58. */
+void orc_ndft_trafo(int d, const int64_t *N, int64_t M, const double *x,
+                    const double *fhat, double *f, const int64_t *rows,
+                    int64_t n_rows, int nthreads)
+{
+    int64_t Nt = prod(N, d);
+    int64_t nout = rows ? n_rows : M;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < nout; ++r) {
+        int64_t j = rows ? rows[r] : r;
+        double sr = 0.0, si = 0.0;
+        int64_t k[MAXD];
+        for (int64_t kl = 0; kl < Nt; ++kl) {
+            k_of(d, N, kl, k);
+            double ph = 0.0;
+            for (int t = 0; t < d; ++t) ph += (double)k[t] * x[d * j + t];
+            double sn, cs;
+            sincos(-2.0 * M_PI * ph, &sn, &cs);
+            sr += fhat[2 * kl] * cs - fhat[2 * kl + 1] * sn;
+            si += fhat[2 * kl] * sn + fhat[2 * kl + 1] * cs;
+        }
+        f[2 * r] = sr;
+        f[2 * r + 1] = si;
+    }
+}
+
+/* kernelNDFT_adjoint_{1,2,3}d / NDFT_adjoint_forElem,
+ * src/cunfft_kernel.cu:1289-1345; NFFT3 adjoint_direct as wrapped by
+ * Tests/Scripts/ndft.cpp:60-113. */
+void orc_ndft_adjoint(int d, const int64_t *N, int64_t M, const double *x,
+                      const double *f, double *fhat, const int64_t *cols,
+                      int64_t n_cols, int nthreads)
+{
+    int64_t Nt = prod(N, d);
+    int64_t nout = cols ? n_cols : Nt;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < nout; ++r) {
+        int64_t kl = cols ? cols[r] : r;
+        int64_t k[MAXD];
+        k_of(d, N, kl, k);
+        double sr = 0.0, si = 0.0;
+        for (int64_t j = 0; j < M; ++j) {
+            double ph = 0.0;
+            for (int t = 0; t < d; ++t) ph += (double)k[t] * x[d * j + t];
+            double sn, cs;
+            sincos(2.0 * M_PI * ph, &sn, &cs);
+            sr += f[2 * j] * cs - f[2 * j + 1] * sn;
+            si += f[2 * j] * sn + f[2 * j + 1] * cs;
+        }
+        fhat[2 * r] = sr;
+        fhat[2 * r + 1] = si;
+    }
+}
+
+/* ------------------------------------------------------------ gradients -- */
+
+/* Derivative of the gather (src/cunfft_kernel.cu:514-1093) with respect to
+ * x_jt, window index set held fixed; phi'(s) = -(2 s / b) phi(s). */
+void orc_gather_dx(int d, const int64_t *N, const int64_t *n, int m, const double *g,
+                   int64_t M, const double *x, double *dfdx)
+{
+    int W = 2 * m + 2;
+    int64_t s1 = d == 3 ? n[2] : (d == 2 ? n[1] : 1);
+    int64_t s0 = d == 3 ? n[1] * n[2] : s1;
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < M; ++j) {
+        struct node_win nw;
+        node_window(d, N, n, m, x + d * j, &nw, 1);
+        for (int tt = 0; tt < d; ++tt) {
+            double sr = 0.0, si = 0.0;
+            if (d == 1) {
+                for (int a = 0; a < W; ++a) {
+                    const double *p = g + 2 * nw.idx[0][a];
+                    sr += p[0] * nw.dw[0][a];
+                    si += p[1] * nw.dw[0][a];
+                }
+            } else if (d == 2) {
+                for (int a = 0; a < W; ++a)
+                    for (int c = 0; c < W; ++c) {
+                        double w = (tt == 0 ? nw.dw[0][a] : nw.w[0][a]) *
+                                   (tt == 1 ? nw.dw[1][c] : nw.w[1][c]);
+                        const double *p = g + 2 * (nw.idx[0][a] * s1 + nw.idx[1][c]);
+                        sr += p[0] * w;
+                        si += p[1] * w;
+                    }
+            } else {
+                for (int a = 0; a < W; ++a)
+                    for (int c = 0; c < W; ++c)
+                        for (int e = 0; e < W; ++e) {
+                            double w = (tt == 0 ? nw.dw[0][a] : nw.w[0][a]) *
+                                       (tt == 1 ? nw.dw[1][c] : nw.w[1][c]) *
+                                       (tt == 2 ? nw.dw[2][e] : nw.w[2][e]);
+                            const double *p = g + 2 * (nw.idx[0][a] * s0 +
+                                                        nw.idx[1][c] * s1 + nw.idx[2][e]);
+                            sr += p[0] * w;
+                            si += p[1] * w;
+                        }
+            }
+            dfdx[2 * (j * d + tt)] = sr;
+            dfdx[2 * (j * d + tt) + 1] = si;
+        }
+    }
+}
+
+/* d/dx_jt of the direct sum of src/cunfft_kernel.cu:1207-1285. */
+void orc_ndft_dx(int d, const int64_t *N, int64_t M, const double *x,
+                 const double *fhat, double *dfdx, const int64_t *rows,
+                 int64_t n_rows, int nthreads)
+{
+    int64_t Nt = prod(N, d);
+    int64_t nout = rows ? n_rows : M;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < nout; ++r) {
+        int64_t j = rows ? rows[r] : r;
+        double sr[MAXD] = {0, 0, 0}, si[MAXD] = {0, 0, 0};
+        int64_t k[MAXD];
+        for (int64_t kl = 0; kl < Nt; ++kl) {
+            k_of(d, N, kl, k);
+            double ph = 0.0;
+            for (int t = 0; t < d; ++t) ph += (double)k[t] * x[d * j + t];
+            double sn, cs;
+            sincos(-2.0 * M_PI * ph, &sn, &cs);
+            /* term = fhat e^{-i theta}; derivative factor -2 pi i k_t */
+            double tr = fhat[2 * kl] * cs - fhat[2 * kl + 1] * sn;
+            double ti = fhat[2 * kl] * sn + fhat[2 * kl + 1] * cs;
+            for (int t = 0; t < d; ++t) {
+                double c = -2.0 * M_PI * (double)k[t];
+                /* (tr + i ti) * (i c) = -c ti + i c tr */
+                sr[t] += -c * ti;
+                si[t] += c * tr;
+            }
+        }
+        for (int t = 0; t < d; ++t) {
+            dfdx[2 * (r * d + t)] = sr[t];
+            dfdx[2 * (r * d + t) + 1] = si[t];
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- norms -- */
+
+/* src/cuda_util.cpp:237-258 (l_infty, compute_error_l_infty). */
+double orc_rel_linf(const double *a, const double *b, int64_t n)
+{
+    double num = 0.0, den = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double dr = a[2 * i] - b[2 * i], di = a[2 * i + 1] - b[2 * i + 1];
+        double e = hypot(dr, di), v = hypot(a[2 * i], a[2 * i + 1]);
+        if (e > num) num = e;
+        if (v > den) den = v;
+    }
+    return num / den;
+}
+
+double orc_rel_l2(const double *a, const double *b, int64_t n)
+{
+    long double num = 0.0L, den = 0.0L;
+    for (int64_t i = 0; i < n; ++i) {
+        long double dr = a[2 * i] - b[2 * i], di = a[2 * i + 1] - b[2 * i + 1];
+        num += dr * dr + di * di;
+        den += (long double)a[2 * i] * a[2 * i] + (long double)a[2 * i + 1] * a[2 * i + 1];
+    }
+    return (double)sqrtl(num / den);
+}

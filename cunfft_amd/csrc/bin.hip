@@ -1,34 +1,516 @@
-// bin.hip -- binned gridding entry points.
+// bin.hip -- node binning and the tiled gridding kernels for gfx950.
+//
+// The reference grids one node per thread straight from global memory: the
+// forward through a texture copy of the grid (src/cunfft_kernel.cu:514-1093,
+// 1408-1418), the adjoint with two CAS-loop atomics per tap into global memory
+// (src/cunfft_kernel.cu:97-130, 608-1197).  Here nodes are first sorted on the
+// device by the grid tile holding their window start (counting sort: keys,
+// histogram, scan, placement), and each workgroup then owns one tile:
+//   forward  the padded tile (T_t + 2m + 1 cells per axis) is staged in LDS
+//            once, and every node of the tile reads its (2m+2)^d taps there;
+//   adjoint  the node contributions accumulate in an LDS copy of the padded
+//            tile with LDS float/double atomics, and the tile is added to the
+//            global grid once, with coalesced global atomics, skipping cells
+//            no node reached.
+// The weights, the window start and the FP64 index/fraction split are the ones
+// of the node-parallel kernels, so both paths compute the same taps.
+#include <hip/amd_detail/amd_hip_unsafe_atomics.h>
+
 #include "bin.hpp"
 
 namespace cunfft {
 
+constexpr int kTileThreads = 256;
+constexpr int kChunk = 256;
+constexpr int kMaxTileBytes = 120 * 1024;
+
+template <int D> struct TileShape;
+template <> struct TileShape<1> { static constexpr int T[3] = {256, 1, 1}; };
+template <> struct TileShape<2> { static constexpr int T[3] = {32, 32, 1}; };
+template <> struct TileShape<3> { static constexpr int T[3] = {8, 8, 8}; };
+
+template <int D, int W> struct Pad {
+    static constexpr int P0 = TileShape<D>::T[0] + W - 1;
+    static constexpr int P1 = D >= 2 ? TileShape<D>::T[1] + W - 1 : 1;
+    static constexpr int P2 = D >= 3 ? TileShape<D>::T[2] + W - 1 : 1;
+    static constexpr int PT = P0 * P1 * P2;
+};
+
+template <typename T, int D, int W> constexpr bool tile_fits()
+{
+    return (size_t)Pad<D, W>::PT * 2 * sizeof(T) <= (size_t)kMaxTileBytes;
+}
+
+struct BinGeom {
+    int d;
+    int logT[3];
+    int64_t nt[3];
+    int64_t ntiles;
+};
+
+static BinGeom bin_geom(const Binning &b)
+{
+    BinGeom g{};
+    g.d = b.d;
+    for (int t = 0; t < 3; ++t) {
+        int l = 0;
+        while ((int64_t(1) << l) < b.T[t]) ++l;
+        g.logT[t] = l;
+        g.nt[t] = b.nt[t];
+    }
+    g.ntiles = b.ntiles;
+    return g;
+}
+
+// ----------------------------------------------------------- binning ----
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
+                                                  uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ tile_count, Geom ge,
+                                                  BinGeom bg)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= ge.M) return;
+    uint32_t key = 0;
+    for (int t = 0; t < bg.d; ++t) {
+        int64_t u;
+        double c;
+        window_start(x[j * bg.d + t], ge.n[t], ge.m, u, c);
+        const uint32_t tc = (uint32_t)((u & (ge.n[t] - 1)) >> bg.logT[t]);
+        key = key * (uint32_t)bg.nt[t] + tc;
+    }
+    keys[j] = key;
+    atomicAdd(&tile_count[key], 1u);
+}
+
+// One workgroup: exclusive scans of the tile counts and of the chunk counts.
+__global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ cnt,
+                                                   uint32_t *__restrict__ start,
+                                                   uint32_t *__restrict__ cursor,
+                                                   uint32_t *__restrict__ chunk_start,
+                                                   uint32_t *__restrict__ n_chunks, int64_t ntiles)
+{
+    __shared__ uint32_t sa[1024], sb[1024];
+    const int tid = threadIdx.x;
+    const int64_t per = (ntiles + 1023) / 1024;
+    const int64_t lo = tid * per, hi = lo + per < ntiles ? lo + per : ntiles;
+    uint32_t a = 0, b = 0;
+    for (int64_t i = lo; i < hi; ++i) {
+        a += cnt[i];
+        b += (cnt[i] + kChunk - 1) / kChunk;
+    }
+    sa[tid] = a;
+    sb[tid] = b;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t va = 0, vb = 0;
+        if (tid >= off) {
+            va = sa[tid - off];
+            vb = sb[tid - off];
+        }
+        __syncthreads();
+        sa[tid] += va;
+        sb[tid] += vb;
+        __syncthreads();
+    }
+    uint32_t ra = sa[tid] - a, rb = sb[tid] - b;
+    for (int64_t i = lo; i < hi; ++i) {
+        const uint32_t c = cnt[i];
+        start[i] = ra;
+        chunk_start[i] = rb;
+        cursor[i] = 0;
+        ra += c;
+        rb += (c + kChunk - 1) / kChunk;
+    }
+    if (tid == 1023) n_chunks[0] = sb[1023];
+}
+
+__global__ __launch_bounds__(256) void k_bin_place(const double *__restrict__ x,
+                                                   const uint32_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ start,
+                                                   uint32_t *__restrict__ cursor,
+                                                   uint32_t *__restrict__ perm,
+                                                   double *__restrict__ xs, int64_t M, int d)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    const uint32_t k = keys[j];
+    const uint32_t pos = start[k] + atomicAdd(&cursor[k], 1u);
+    perm[pos] = (uint32_t)j;
+    for (int t = 0; t < d; ++t) xs[(int64_t)pos * d + t] = x[j * d + t];
+}
+
+__global__ __launch_bounds__(256) void k_bin_chunks(const uint32_t *__restrict__ cnt,
+                                                    const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ chunk_start,
+                                                    Chunk *__restrict__ chunks, int64_t ntiles)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ntiles) return;
+    const uint32_t c = cnt[i];
+    uint32_t o = chunk_start[i];
+    for (uint32_t s = 0; s < c; s += kChunk, ++o) {
+        Chunk ch;
+        ch.tile = (uint32_t)i;
+        ch.start = start[i] + s;
+        ch.count = c - s < (uint32_t)kChunk ? c - s : (uint32_t)kChunk;
+        ch.pad = 0;
+        chunks[o] = ch;
+    }
+}
+
+bool Binning::plan(const Geom &ge, int elem_bytes)
+{
+    d = ge.d;
+    usable = 0;
+    const int W = ge.W;
+    if (W > 18 || ge.M >= (int64_t(1) << 31)) return false;
+    static const int shapes[4][3] = {{1, 1, 1}, {256, 1, 1}, {32, 32, 1}, {8, 8, 8}};
+    int64_t bytes = 2 * elem_bytes;
+    ntiles = 1;
+    for (int t = 0; t < 3; ++t) {
+        T[t] = t < d ? shapes[d][t] : 1;
+        if (t < d && ge.n[t] < T[t]) return false;
+        nt[t] = t < d ? ge.n[t] / T[t] : 1;
+        ntiles *= nt[t];
+        if (t < d) bytes *= T[t] + W - 1;
+    }
+    if (bytes > kMaxTileBytes) return false;
+    if (d == 3 && W > 14) return false;
+    if (ntiles >= (int64_t(1) << 31)) return false;
+    usable = 1;
+    return true;
+}
+
+static void *dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    CUNFFT_HIP(hipMalloc(&p, bytes ? bytes : 16));
+    return p;
+}
+
 void Binning::build(const Geom &ge, const double *x, hipStream_t s)
 {
-    (void)ge;
-    (void)x;
-    (void)s;
+    if (!usable) return;
+    if (!built || M != ge.M) {
+        release();
+        M = ge.M;
+        max_chunks = ntiles + (M + kChunk - 1) / kChunk;
+        keys = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+        perm = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+        tile_count = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
+        tile_start = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
+        cursor = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
+        chunk_start = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
+        n_chunks = (uint32_t *)dev_alloc(sizeof(uint32_t));
+        chunks = (Chunk *)dev_alloc(sizeof(Chunk) * max_chunks);
+        xs = (double *)dev_alloc(sizeof(double) * M * d);
+        M = ge.M;
+    }
+    const BinGeom bg = bin_geom(*this);
+    k_fill_u32<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, ntiles, 0u);
+    k_bin_keys<<<(unsigned)((M + 255) / 256), 256, 0, s>>>(x, keys, tile_count, ge, bg);
+    k_bin_scan<<<1, 1024, 0, s>>>(tile_count, tile_start, cursor, chunk_start, n_chunks, ntiles);
+    k_bin_place<<<(unsigned)((M + 255) / 256), 256, 0, s>>>(x, keys, tile_start, cursor, perm, xs,
+                                                            M, d);
+    k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
+                                                                  chunk_start, chunks, ntiles);
+    CUNFFT_HIP(hipGetLastError());
     built = 1;
 }
 
-void Binning::release() { built = 0; }
+void Binning::release()
+{
+    if (keys) (void)hipFree(keys);
+    if (perm) (void)hipFree(perm);
+    if (tile_count) (void)hipFree(tile_count);
+    if (tile_start) (void)hipFree(tile_start);
+    if (cursor) (void)hipFree(cursor);
+    if (chunk_start) (void)hipFree(chunk_start);
+    if (n_chunks) (void)hipFree(n_chunks);
+    if (chunks) (void)hipFree(chunks);
+    if (xs) (void)hipFree(xs);
+    keys = perm = tile_count = tile_start = cursor = chunk_start = n_chunks = nullptr;
+    chunks = nullptr;
+    xs = nullptr;
+    built = 0;
+    M = 0;
+}
+
+// ------------------------------------------------------ tile helpers ----
+template <int D>
+__device__ __forceinline__ void tile_origin(uint32_t tile, const BinGeom &bg, int64_t (&o)[3])
+{
+    uint32_t rem = tile;
+#pragma unroll
+    for (int t = D - 1; t >= 0; --t) {
+        const uint32_t tc = rem % (uint32_t)bg.nt[t];
+        rem /= (uint32_t)bg.nt[t];
+        o[t] = (int64_t)tc << bg.logT[t];
+    }
+#pragma unroll
+    for (int t = D; t < 3; ++t) o[t] = 0;
+}
+
+// Global grid offset of padded-tile cell idx.
+template <int D, int W>
+__device__ __forceinline__ int64_t tile_cell(int idx, const int64_t (&o)[3], const Geom &ge)
+{
+    using PD = Pad<D, W>;
+    const int p0 = idx / (PD::P1 * PD::P2);
+    const int r = idx - p0 * (PD::P1 * PD::P2);
+    const int p1 = r / PD::P2;
+    const int p2 = r - p1 * PD::P2;
+    const int64_t c0 = (o[0] + p0) & (ge.n[0] - 1);
+    if constexpr (D == 1) return c0;
+    const int64_t c1 = (o[1] + p1) & (ge.n[1] - 1);
+    if constexpr (D == 2) return c0 * ge.n[1] + c1;
+    const int64_t c2 = (o[2] + p2) & (ge.n[2] - 1);
+    return (c0 * ge.n[1] + c1) * ge.n[2] + c2;
+}
+
+// Window weights of the node at sorted position s and its LDS base offset.
+template <typename T, int D, int W>
+__device__ __forceinline__ int node_local(const double *__restrict__ xs, uint32_t s,
+                                          const int64_t (&o)[3], const Geom &ge, T (&w)[D][W])
+{
+    using PD = Pad<D, W>;
+    constexpr int stride[3] = {PD::P1 * PD::P2, PD::P2, 1};
+    int base = 0;
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+        int64_t u;
+        double c;
+        window_start(xs[(int64_t)s * D + t], ge.n[t], ge.m, u, c);
+        int lo = (int)((u & (ge.n[t] - 1)) - o[t]);
+        lo = lo < 0 ? 0 : (lo >= TileShape<D>::T[t] ? TileShape<D>::T[t] - 1 : lo);
+        window_weights<T, W>(c, ge.invb[t], ge.nrm[t], w[t]);
+        base += lo * stride[t];
+    }
+    return base;
+}
+
+// --------------------------------------------------------- tiled gather --
+template <typename T, int D, int W>
+__global__ __launch_bounds__(kTileThreads) void k_gather_tiled(
+    const cplx_t<T> *__restrict__ g, const double *__restrict__ xs, const uint32_t *__restrict__ perm,
+    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f,
+    Geom ge, BinGeom bg)
+{
+    using PD = Pad<D, W>;
+    __shared__ cplx_t<T> tile[PD::PT];
+    if (blockIdx.x >= n_chunks[0]) return;
+    const Chunk ch = chunks[blockIdx.x];
+    int64_t o[3];
+    tile_origin<D>(ch.tile, bg, o);
+    for (int idx = threadIdx.x; idx < PD::PT; idx += kTileThreads)
+        tile[idx] = g[tile_cell<D, W>(idx, o, ge)];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ch.count; i += kTileThreads) {
+        const uint32_t s = ch.start + i;
+        T w[D][W];
+        const int base = node_local<T, D, W>(xs, s, o, ge, w);
+        const cplx_t<T> *tb = tile + base;
+        T sr = 0, si = 0;
+        if constexpr (D == 1) {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                const cplx_t<T> v = tb[a];
+                sr += v.x * w[0][a];
+                si += v.y * w[0][a];
+            }
+        } else if constexpr (D == 2) {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                T r1 = 0, i1 = 0;
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    const cplx_t<T> v = tb[a * PD::P1 + c];
+                    r1 += v.x * w[1][c];
+                    i1 += v.y * w[1][c];
+                }
+                sr += r1 * w[0][a];
+                si += i1 * w[0][a];
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                T r1 = 0, i1 = 0;
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    T r2 = 0, i2 = 0;
+#pragma unroll
+                    for (int e = 0; e < W; ++e) {
+                        const cplx_t<T> v = tb[(a * PD::P1 + c) * PD::P2 + e];
+                        r2 += v.x * w[2][e];
+                        i2 += v.y * w[2][e];
+                    }
+                    r1 += r2 * w[1][c];
+                    i1 += i2 * w[1][c];
+                }
+                sr += r1 * w[0][a];
+                si += i1 * w[0][a];
+            }
+        }
+        f[perm[s]] = cmake<T>(sr, si);
+    }
+}
+
+// -------------------------------------------------------- tiled scatter --
+template <typename T>
+__device__ __forceinline__ void lds_add(T *p, T v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <typename T, int D, int W>
+__global__ __launch_bounds__(kTileThreads) void k_scatter_tiled(
+    const double *__restrict__ xs, const uint32_t *__restrict__ perm, const cplx_t<T> *__restrict__ f,
+    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ g,
+    Geom ge, BinGeom bg)
+{
+    using PD = Pad<D, W>;
+    __shared__ T acc_re[PD::PT];
+    __shared__ T acc_im[PD::PT];
+    if (blockIdx.x >= n_chunks[0]) return;
+    const Chunk ch = chunks[blockIdx.x];
+    int64_t o[3];
+    tile_origin<D>(ch.tile, bg, o);
+    for (int idx = threadIdx.x; idx < PD::PT; idx += kTileThreads) {
+        acc_re[idx] = 0;
+        acc_im[idx] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ch.count; i += kTileThreads) {
+        const uint32_t s = ch.start + i;
+        T w[D][W];
+        const int base = node_local<T, D, W>(xs, s, o, ge, w);
+        const cplx_t<T> fj = f[perm[s]];
+        T *re = acc_re + base;
+        T *im = acc_im + base;
+        if constexpr (D == 1) {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                lds_add<T>(re + a, fj.x * w[0][a]);
+                lds_add<T>(im + a, fj.y * w[0][a]);
+            }
+        } else if constexpr (D == 2) {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                const T fr = fj.x * w[0][a], fi = fj.y * w[0][a];
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    lds_add<T>(re + a * PD::P1 + c, fr * w[1][c]);
+                    lds_add<T>(im + a * PD::P1 + c, fi * w[1][c]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                const T fr0 = fj.x * w[0][a], fi0 = fj.y * w[0][a];
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    const T fr = fr0 * w[1][c], fi = fi0 * w[1][c];
+#pragma unroll
+                    for (int e = 0; e < W; ++e) {
+                        const int off = (a * PD::P1 + c) * PD::P2 + e;
+                        lds_add<T>(re + off, fr * w[2][e]);
+                        lds_add<T>(im + off, fi * w[2][e]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < PD::PT; idx += kTileThreads) {
+        const T vr = acc_re[idx], vi = acc_im[idx];
+        if (vr != (T)0 || vi != (T)0) {
+            cplx_t<T> *p = g + tile_cell<D, W>(idx, o, ge);
+            unsafeAtomicAdd(&p->x, vr);
+            unsafeAtomicAdd(&p->y, vi);
+        }
+    }
+}
+
+// ------------------------------------------------------------ dispatch --
+template <typename T, int D, int W>
+static void gather_tiled(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
+                         hipStream_t s)
+{
+    if constexpr (tile_fits<T, D, W>()) {
+        k_gather_tiled<T, D, W><<<(unsigned)b.max_chunks, kTileThreads, 0, s>>>(
+            g, b.xs, b.perm, b.chunks, b.n_chunks, f, ge, bin_geom(b));
+    } else {
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
+    }
+}
+
+template <typename T, int D, int W>
+static void scatter_tiled(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
+                          hipStream_t s)
+{
+    if constexpr (tile_fits<T, D, W>()) {
+        k_scatter_tiled<T, D, W><<<(unsigned)b.max_chunks, kTileThreads, 0, s>>>(
+            b.xs, b.perm, f, b.chunks, b.n_chunks, g, ge, bin_geom(b));
+    } else {
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
+    }
+}
+
+#define CUNFFT_TILED_W(FN, D)                     \
+    switch (ge.W) {                               \
+    case 4: FN<T, D, 4>(ge, b, a0, a1, s); break; \
+    case 6: FN<T, D, 6>(ge, b, a0, a1, s); break; \
+    case 8: FN<T, D, 8>(ge, b, a0, a1, s); break; \
+    case 10: FN<T, D, 10>(ge, b, a0, a1, s); break; \
+    case 12: FN<T, D, 12>(ge, b, a0, a1, s); break; \
+    case 14: FN<T, D, 14>(ge, b, a0, a1, s); break; \
+    case 16: if constexpr (D < 3) FN<T, D, 16>(ge, b, a0, a1, s); break; \
+    case 18: if constexpr (D < 3) FN<T, D, 18>(ge, b, a0, a1, s); break; \
+    default: throw Error(CUNFFT_ERR_UNSUPPORTED, "tiled gridding needs m <= 8"); \
+    }
 
 template <typename T>
 void launch_gather_binned(const Geom &ge, const Binning &b, const cplx_t<T> *g, const double *x,
                           cplx_t<T> *f, hipStream_t s)
 {
-    (void)b;
-    launch_gather_plain<T>(ge, g, x, f, s);
+    if (!b.usable || !b.built) {
+        launch_gather_plain<T>(ge, g, x, f, s);
+        return;
+    }
+    const cplx_t<T> *a0 = g;
+    cplx_t<T> *a1 = f;
+    switch (ge.d) {
+    case 1: CUNFFT_TILED_W(gather_tiled, 1) break;
+    case 2: CUNFFT_TILED_W(gather_tiled, 2) break;
+    default: CUNFFT_TILED_W(gather_tiled, 3) break;
+    }
+    CUNFFT_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, const cplx_t<T> *f,
                            cplx_t<T> *g, hipStream_t s)
 {
-    (void)b;
-    int64_t nt = ge.n[0] * ge.n[1] * ge.n[2];
+    const int64_t nt = ge.n[0] * ge.n[1] * ge.n[2];
     launch_zero<T>(g, nt, s);
-    launch_scatter_atomic<T>(ge, x, f, g, s);
+    if (!b.usable || !b.built) {
+        launch_scatter_atomic<T>(ge, x, f, g, s);
+        return;
+    }
+    const cplx_t<T> *a0 = f;
+    cplx_t<T> *a1 = g;
+    switch (ge.d) {
+    case 1: CUNFFT_TILED_W(scatter_tiled, 1) break;
+    case 2: CUNFFT_TILED_W(scatter_tiled, 2) break;
+    default: CUNFFT_TILED_W(scatter_tiled, 3) break;
+    }
+    CUNFFT_HIP(hipGetLastError());
 }
 
 template void launch_gather_binned<float>(const Geom &, const Binning &, const float2 *,

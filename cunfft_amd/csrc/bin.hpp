@@ -1,14 +1,43 @@
-// bin.hpp -- node binning state shared by the tiled gridding kernels.
+// bin.hpp -- node binning shared by the tiled gridding kernels.
 #pragma once
 #include "internal.hpp"
 
 namespace cunfft {
 
-// Nodes sorted by the grid tile that holds their window start.  Built once per
-// set of nodes on the plan stream and reused by every forward and adjoint
-// call until the nodes change.
+// Work item of the tiled kernels: up to kChunk sorted nodes of one tile.
+struct Chunk {
+    uint32_t tile;
+    uint32_t start;  // first sorted position
+    uint32_t count;
+    uint32_t pad;
+};
+
+// Nodes sorted (counting sort on the device) by the grid tile that holds the
+// start of their window, u_t mod n_t.  Built once per node set on the plan
+// stream and reused by every forward and adjoint call until the nodes change.
+// Tiles with many nodes are split into chunks so one workgroup never owns more
+// than kChunk nodes.
 struct Binning {
     int built = 0;
+    int usable = 0;             // tiled kernels apply to this geometry
+    int d = 0;
+    int64_t T[3] = {1, 1, 1};   // tile extent per axis
+    int64_t nt[3] = {1, 1, 1};  // tiles per axis
+    int64_t ntiles = 0;
+    int64_t M = 0;
+    int64_t max_chunks = 0;
+    uint32_t *keys = nullptr;       // M: tile of each node
+    uint32_t *perm = nullptr;       // M: node at each sorted position
+    uint32_t *tile_count = nullptr; // ntiles
+    uint32_t *tile_start = nullptr; // ntiles
+    uint32_t *cursor = nullptr;     // ntiles
+    uint32_t *chunk_start = nullptr;// ntiles
+    uint32_t *n_chunks = nullptr;   // 1
+    Chunk *chunks = nullptr;        // max_chunks
+    double *xs = nullptr;           // M*d: nodes in sorted order
+
+    // Decide tile shape for this geometry; false if the tiled path does not apply.
+    bool plan(const Geom &ge, int elem_bytes);
     void build(const Geom &ge, const double *x, hipStream_t s);
     void release();
 };

@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "cunfft.h"
 #include "device_common.hpp"
 
 namespace cunfft {

@@ -256,6 +256,7 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
         ge.invb[t] = t < d ? 1.0 / p->b[t] : 0.0;
         ge.nrm[t] = t < d ? 1.0 / sqrt(M_PI * p->b[t]) : 0.0;
     }
+    im->bins.plan(ge, (int)sizeof(T));
     // roll-off tables by pruned index p: exp(fac_t k^2), k = p or p - N_t
     std::vector<double> roc;
     for (int t = 0; t < d; ++t) {

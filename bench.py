@@ -71,31 +71,42 @@ def time_plan(cu, precision, x, fh, fv, steps, warmup):
     p.copyDataToDevice()
     p.copyDataToDeviceAd()
     torch.cuda.synchronize()
-    # timed region: K steps of forward + adjoint, device-resident data
+    # timed region: K steps of forward + adjoint, device-resident data, binning warm
     t0 = time.perf_counter()
     for _ in range(steps):
         p.transform_dev(dp["f_hat"], dp["f"])
         p.adjoint_dev(dp["f"], dp["f_hat"])
     torch.cuda.synchronize()  # device-wide: waits for the plan's own stream too
-    t_total = time.perf_counter() - t0
-    ms_per_step = 1e3 * t_total / steps
-    # per-step device times with HIP events on the plan stream
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / steps
+    # cold binning: the nodes are re-set (and re-sorted) before every step
+    cold_steps = max(2, steps // 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(cold_steps):
+        p.set_nodes_dev(dp["x"])
+        p.transform_dev(dp["f_hat"], dp["f"])
+        p.adjoint_dev(dp["f"], dp["f_hat"])
+    torch.cuda.synchronize()
+    cold_ms = 1e3 * (time.perf_counter() - t0) / cold_steps
+    # per-kernel device times with HIP events on the plan stream
     p._p.flags = p._p.flags | cu.MEASURE_TIME
-    fw_conv, fw_fft, ad_conv, ad_fft = [], [], [], []
+    rec = {k: [] for k in ("fwd_fft", "fwd_gather", "adj_grid", "adj_fold", "adj_fft")}
     for _ in range(max(3, min(steps, 10))):
         p.transform()
         t = p.times
-        fw_conv.append(t["time_CONV"])
-        fw_fft.append(t["time_FFT"])
+        rec["fwd_fft"].append(t["time_FFT"])
+        rec["fwd_gather"].append(t["time_CONV"])
         p.adjoint()
         t = p.times
-        ad_conv.append(t["time_CONV"])
-        ad_fft.append(t["time_FFT"])
+        rec["adj_grid"].append(t["time_CONV"])
+        rec["adj_fold"].append(t["time_else"])
+        rec["adj_fft"].append(t["time_FFT"])
     p._p.flags = p._p.flags & ~cu.MEASURE_TIME
-    med = lambda v: float(np.median(v))
-    return p, {"ms_per_step": ms_per_step, "fwd_gather_ms": 1e3 * med(fw_conv),
-               "fwd_fft_ms": 1e3 * med(fw_fft), "adj_scatter_ms": 1e3 * med(ad_conv),
-               "adj_fft_ms": 1e3 * med(ad_fft)}
+    med = {k: 1e3 * float(np.median(v)) for k, v in rec.items()}
+    return p, {"ms_per_step": ms_per_step, "cold_binning_ms_per_step": cold_ms,
+               "fwd_fft_ms": med["fwd_fft"], "fwd_gather_ms": med["fwd_gather"],
+               "adj_scatter_ms": med["adj_grid"] - med["adj_fold"], "adj_fold_ms": med["adj_fold"],
+               "adj_fft_ms": med["adj_fft"]}
 
 
 def accuracy(cu, p32, p64, x, fh, n_sample=4096):
@@ -135,6 +146,22 @@ def accuracy(cu, p32, p64, x, fh, n_sample=4096):
     return out
 
 
+def measured_traffic(kern):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/), or None if it is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_gridding_3d128_fp32.txt")
+    key = {"fwd_gather_ms": "k_gather_rows", "adj_scatter_ms": "k_scatter_layers"}[kern]
+    try:
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) >= 2 and parts[0] == key + "_bytes_per_launch":
+                    return float(parts[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(x, fh, budget_s=15.0):
     """The oracle's FP64 direct NDFT (OpenMP, all host cores) on sampled rows of
     the same workload; throughput extrapolated linearly to the full fwd+adj NDFT."""
@@ -170,7 +197,20 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (to scale HBM "
+                         "counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+
+        src = torch.empty(a.copy_calibration // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4}))
+        return 0
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "the multi-GPU sharding is a design, not a run", file=sys.stderr)
@@ -196,12 +236,14 @@ def main():
 
     value = M_NODES / (t32["ms_per_step"] * 1e-3)
     value64 = M_NODES / (t64["ms_per_step"] * 1e-3)
-    # dominant kernel of the FP32 step and its roofline
-    kern = max(("fwd_gather_ms", "adj_scatter_ms"), key=lambda k: t32[k])
+    # dominant kernel of the FP32 step and its roofline (algorithmic bytes of one
+    # gridding launch: the grid once plus the nodes, see grid_bytes)
+    names = {"fwd_gather_ms": "k_gather_rows<float,3,10> (forward gridding)",
+             "adj_scatter_ms": "k_scatter_layers<float,3,10> (adjoint tile accumulation)"}
+    kern = max(names, key=lambda k: t32[k])
     ach = grid_bytes(8) / (t32[kern] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "gather (forward gridding)" if kern == "fwd_gather_ms"
-            else "scatter (adjoint gridding)", "achieved": ach, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+    roof = {"bound": "hbm", "kernel": names[kern], "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(kern),
             "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
     cpu = None
     if not a.no_cpu_baseline:

@@ -23,7 +23,7 @@ import numpy as np
 
 __all__ = ["Plan", "CunfftError", "lib", "LIB_PATH", "CUFFT_INIT", "CUFFT_BOTH",
            "CUFFT_FORW", "CUFFT_BACKW", "MEASURE_TIME", "NO_HOST_BUFFERS",
-           "ATOMIC_SCATTER", "PLAIN_GATHER", "default_n"]
+           "ATOMIC_SCATTER", "PLAIN_GATHER", "NODE_LANES", "default_n"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcunfft_amd.so")
 
@@ -35,6 +35,7 @@ MEASURE_TIME = 1 << 8
 NO_HOST_BUFFERS = 1 << 9
 ATOMIC_SCATTER = 1 << 10
 PLAIN_GATHER = 1 << 11
+NODE_LANES = 1 << 12
 DEFAULT_CUT_OFF = 6
 
 _STATUS = {0: "success", 1: "invalid argument", 2: "unsupported parameters",
@@ -92,7 +93,11 @@ _PLAN_VOID = {
     "finalizeDevice": ("cunfft_finalizeDevice", "cunfftf_finalizeDevice"),
     "initDevicePara": ("cunfft_initDevicePara", "cunfftf_initDevicePara"),
     "freePlan": ("cunfft_freePlan", "cunfftf_freePlan"),
+    "copy_f_ToHost": ("copy_f_ToHost", "cunfftf_copy_f_ToHost"),
+    "copy_f_hat_ToHost": ("copy_f_hat_ToHost", "cunfftf_copy_f_hat_ToHost"),
+    "copy_x_toHost": ("copy_x_toHost", "cunfftf_copy_x_toHost"),
 }
+_PLAN_PTR_ARG = ("copy_f_ToHost", "copy_f_hat_ToHost", "copy_x_toHost")
 
 
 def lib():
@@ -105,9 +110,9 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     PP = ctypes.POINTER(cunfft_plan)
     vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_ssize_t)
-    for a, b in _PLAN_VOID.values():
+    for k, (a, b) in _PLAN_VOID.items():
         for s in (a, b):
-            getattr(L, s).argtypes = [PP]
+            getattr(L, s).argtypes = [PP, vp] if k in _PLAN_PTR_ARG else [PP]
             getattr(L, s).restype = None
     for s in ("cunfft_finalize", "cunfftf_finalize"):
         getattr(L, s).argtypes = [PP]
@@ -193,7 +198,7 @@ class Plan:
             raise CunfftError(st, msg)
 
     def _call(self, name, *args):
-        a, b = _PLAN_VOID[name] if name in _PLAN_VOID else (name, name)
+        a, b = _PLAN_VOID[name]
         getattr(lib(), b if self._sp else a)(ctypes.byref(self._p), *args)
         self._check()
 

@@ -20,6 +20,7 @@ struct Chunk {
 struct Binning {
     int built = 0;
     int usable = 0;             // tiled kernels apply to this geometry
+    int variant = 0;            // 0: lanes along the window row (d >= 2); 1: one node per lane
     int d = 0;
     int64_t T[3] = {1, 1, 1};   // tile extent per axis
     int64_t nt[3] = {1, 1, 1};  // tiles per axis
@@ -35,9 +36,11 @@ struct Binning {
     uint32_t *n_chunks = nullptr;   // 1
     Chunk *chunks = nullptr;        // max_chunks
     double *xs = nullptr;           // M*d: nodes in sorted order
+    void *slots = nullptr;          // max_chunks padded tiles (adjoint partial sums)
+    int slot_elem_bytes = 8;        // bytes of one real component of a slot cell
 
     // Decide tile shape for this geometry; false if the tiled path does not apply.
-    bool plan(const Geom &ge, int elem_bytes);
+    bool plan(const Geom &ge, int elem_bytes, int kernel_variant);
     void build(const Geom &ge, const double *x, hipStream_t s);
     void release();
 };
@@ -45,8 +48,11 @@ struct Binning {
 template <typename T>
 void launch_gather_binned(const Geom &ge, const Binning &b, const cplx_t<T> *g, const double *x,
                           cplx_t<T> *f, hipStream_t s);
+// mid (optional) is recorded on s after the tile accumulation and before the
+// fold, so callers can time the two adjoint gridding kernels separately.
+// Returns true if mid was recorded (the path has a fold).
 template <typename T>
-void launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, const cplx_t<T> *f,
-                           cplx_t<T> *g, hipStream_t s);
+bool launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, const cplx_t<T> *f,
+                           cplx_t<T> *g, hipStream_t s, hipEvent_t mid = nullptr);
 
 }  // namespace cunfft

@@ -256,7 +256,7 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
         ge.invb[t] = t < d ? 1.0 / p->b[t] : 0.0;
         ge.nrm[t] = t < d ? 1.0 / sqrt(M_PI * p->b[t]) : 0.0;
     }
-    im->bins.plan(ge, (int)sizeof(T));
+    im->bins.plan(ge, (int)sizeof(T), (p->flags & CUNFFT_NODE_LANES) ? 1 : 0);
     // roll-off tables by pruned index p: exp(fac_t k^2), k = p or p - N_t
     std::vector<double> roc;
     for (int t = 0; t < d; ++t) {
@@ -414,32 +414,39 @@ static void run_forward(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
     }
 }
 
-// fhat = A^H f.
+// fhat = A^H f.  Timing: time_CONV = adjoint gridding (tile accumulation and
+// fold), time_else = the fold alone, time_FFT = the FFT passes.
 template <typename T>
 static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T> *f,
                         cplx_t<T> *fhat, hipStream_t s, bool timing, double *t_fft,
-                        double *t_conv)
+                        double *t_conv, double *t_fold)
 {
     cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
     const double *x = p->x_gpu;
-    if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+    bool folded = false;
     if (p->flags & CUNFFT_ATOMIC_SCATTER) {
+        if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
         launch_zero<T>(g, p->n_total, s);
         launch_scatter_atomic<T>(im->geom, x, f, g, s);
     } else {
         ensure_bins<T>(p, im, x, s);
-        launch_scatter_binned<T>(im->geom, im->bins, x, f, g, s);
+        if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+        folded = launch_scatter_binned<T>(im->geom, im->bins, x, f, g, s,
+                                          timing ? im->ev[3] : nullptr);
     }
+    if (timing && !folded) CUNFFT_HIP(hipEventRecord(im->ev[3], s));
     if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], s));
     fft_passes<T>(p, im, fft_tables<T>(p, false), false, nullptr, fhat, g, s);
     if (timing) {
         CUNFFT_HIP(hipEventRecord(im->ev[2], s));
         CUNFFT_HIP(hipEventSynchronize(im->ev[2]));
-        float a = 0, b = 0;
+        float a = 0, b = 0, c = 0;
         CUNFFT_HIP(hipEventElapsedTime(&a, im->ev[0], im->ev[1]));
         CUNFFT_HIP(hipEventElapsedTime(&b, im->ev[1], im->ev[2]));
+        CUNFFT_HIP(hipEventElapsedTime(&c, im->ev[3], im->ev[1]));
         *t_conv = a * 1e-3;
         *t_fft = b * 1e-3;
+        *t_fold = c * 1e-3;
     }
 }
 
@@ -601,13 +608,15 @@ template <typename T> void api_adjoint(typename Traits<T>::plan *p)
         Impl<T> *im = impl_of<T>(p);
         need_device<T>(p);
         const bool timing = p->flags & CUNFFT_MEASURE_TIME;
-        double tf = 0, tc = 0;
+        double tf = 0, tc = 0, tfold = 0;
         run_adjoint<T>(p, im, reinterpret_cast<const cplx_t<T> *>(p->f_gpu),
-                       reinterpret_cast<cplx_t<T> *>(p->fhat_gpu), im->stream, timing, &tf, &tc);
+                       reinterpret_cast<cplx_t<T> *>(p->fhat_gpu), im->stream, timing, &tf, &tc,
+                       &tfold);
         CUNFFT_HIP(hipStreamSynchronize(im->stream));
         p->CUNFFTTimes.time_ROC = 0.0;
         p->CUNFFTTimes.time_FFT = tf;
         p->CUNFFTTimes.time_CONV = tc;
+        p->CUNFFTTimes.time_else = tfold;
         p->CUNFFTTimes.runTime = tf + tc;
     });
 }
@@ -689,9 +698,9 @@ int api_adjoint_dev(typename Traits<T>::plan *p, const void *f, void *fhat, void
         Impl<T> *im = impl_of<T>(p);
         need_device<T>(p);
         if (!fhat || !f) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
-        double a, b;
+        double a, b, c;
         run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(f), static_cast<cplx_t<T> *>(fhat),
-                       pick_stream(stream, im->stream), false, &a, &b);
+                       pick_stream(stream, im->stream), false, &a, &b, &c);
     });
 }
 

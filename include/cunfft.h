@@ -61,6 +61,9 @@ extern "C" {
 #define CUNFFT_ATOMIC_SCATTER (1U << 10)
 /** Forward gridding over unsorted nodes (one node per lane, plain path). */
 #define CUNFFT_PLAIN_GATHER   (1U << 11)
+/** Tiled gridding with one node per lane instead of one lane per tap of the
+ *  window row (d >= 2). */
+#define CUNFFT_NODE_LANES     (1U << 12)
 
 /* status codes */
 #define CUNFFT_SUCCESS          0

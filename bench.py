@@ -150,7 +150,7 @@ def measured_traffic(kern):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (profiles/), or None if it is absent."""
     path = os.path.join(ROOT, "profiles", "traffic_gridding_3d128_fp32.txt")
-    key = {"fwd_gather_ms": "k_gather_rows", "adj_scatter_ms": "k_scatter_layers"}[kern]
+    key = {"fwd_gather_ms": "k_tile_gather", "adj_scatter_ms": "k_tile_spread"}[kern]
     try:
         with open(path) as fh:
             for line in fh:
@@ -238,8 +238,9 @@ def main():
     value64 = M_NODES / (t64["ms_per_step"] * 1e-3)
     # dominant kernel of the FP32 step and its roofline (algorithmic bytes of one
     # gridding launch: the grid once plus the nodes, see grid_bytes)
-    names = {"fwd_gather_ms": "k_gather_rows<float,3,10> (forward gridding)",
-             "adj_scatter_ms": "k_scatter_layers<float,3,10> (adjoint tile accumulation)"}
+    names = {"fwd_gather_ms": "k_tile_gather<float,3,10,4> (forward gridding)",
+             "adj_scatter_ms": "k_tile_spread<float,3,10,2> (adjoint tile accumulation, "
+                               "with the sorted-coefficient copy)"}
     kern = max(names, key=lambda k: t32[k])
     ach = grid_bytes(8) / (t32[kern] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": names[kern], "achieved": ach, "peak": HBM_PEAK_GBS,

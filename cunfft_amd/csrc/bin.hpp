@@ -4,12 +4,37 @@
 
 namespace cunfft {
 
+constexpr int kChunk = 256;   // nodes per chunk (one workgroup's share of a tile)
+
+// Grid tile per dimension: the nodes whose window starts in one tile are
+// gridded by one workgroup against the tile plus a (2m+1)-cell halo in LDS.
+template <int D> struct TileShape;
+template <> struct TileShape<1> { static constexpr int T[3] = {256, 1, 1}; };
+template <> struct TileShape<2> { static constexpr int T[3] = {32, 32, 1}; };
+template <> struct TileShape<3> { static constexpr int T[3] = {8, 8, 8}; };
+
 // Work item of the tiled kernels: up to kChunk sorted nodes of one tile.
 struct Chunk {
     uint32_t tile;
     uint32_t start;  // first sorted position
     uint32_t count;
     uint32_t pad;
+};
+
+// Per-node window parameters in sorted order, written at binning time so the
+// gridding kernels do no FP64 index arithmetic:
+//   f[t]  = c_t - m, with c_t = n_t x_t - u_t the offset of the node from its
+//           first window cell u_t (window tap l has argument f[t] + m - l);
+//   loc   = tile-local first window cell, lo_0 | lo_1 << 8 | lo_2 << 16.
+// FP32 plans keep f in FP32 (|f| < 2, so the rounding is below 1.2e-7).
+template <typename T> struct NodeDesc;
+template <> struct alignas(16) NodeDesc<float> {
+    float f[3];
+    uint32_t loc;
+};
+template <> struct alignas(16) NodeDesc<double> {
+    double f[3];
+    uint64_t loc;
 };
 
 // Nodes sorted (counting sort on the device) by the grid tile that holds the
@@ -20,13 +45,14 @@ struct Chunk {
 struct Binning {
     int built = 0;
     int usable = 0;             // tiled kernels apply to this geometry
-    int variant = 0;            // 0: lanes along the window row (d >= 2); 1: one node per lane
+    int variant = 0;            // 0: plane-tiled kernels (d >= 2); 1: one node per lane
     int d = 0;
     int64_t T[3] = {1, 1, 1};   // tile extent per axis
     int64_t nt[3] = {1, 1, 1};  // tiles per axis
     int64_t ntiles = 0;
     int64_t M = 0;
     int64_t max_chunks = 0;
+    int64_t slot_cells = 0;     // padded-tile cells of one adjoint slot
     uint32_t *keys = nullptr;       // M: tile of each node
     uint32_t *perm = nullptr;       // M: node at each sorted position
     uint32_t *tile_count = nullptr; // ntiles
@@ -35,15 +61,26 @@ struct Binning {
     uint32_t *chunk_start = nullptr;// ntiles
     uint32_t *n_chunks = nullptr;   // 1
     Chunk *chunks = nullptr;        // max_chunks
-    double *xs = nullptr;           // M*d: nodes in sorted order
+    double *xs = nullptr;           // M*d: nodes in sorted order (node-lane kernels)
+    void *desc = nullptr;           // M: NodeDesc<T> in sorted order (plane-tiled kernels)
     void *slots = nullptr;          // max_chunks padded tiles (adjoint partial sums)
-    int slot_elem_bytes = 8;        // bytes of one real component of a slot cell
+    void *fs = nullptr;             // M: adjoint coefficients in sorted order
+    int elem_bytes = 8;             // bytes of one real component (4: FP32 plan, 8: FP64)
 
     // Decide tile shape for this geometry; false if the tiled path does not apply.
     bool plan(const Geom &ge, int elem_bytes, int kernel_variant);
     void build(const Geom &ge, const double *x, hipStream_t s);
     void release();
+    bool planes() const { return d >= 2 && variant == 0; }
 };
+
+struct BinGeom {
+    int d;
+    int logT[3];
+    int64_t nt[3];
+    int64_t ntiles;
+};
+BinGeom bin_geom(const Binning &b);
 
 template <typename T>
 void launch_gather_binned(const Geom &ge, const Binning &b, const cplx_t<T> *g, const double *x,
@@ -54,5 +91,15 @@ void launch_gather_binned(const Geom &ge, const Binning &b, const cplx_t<T> *g, 
 template <typename T>
 bool launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, const cplx_t<T> *f,
                            cplx_t<T> *g, hipStream_t s, hipEvent_t mid = nullptr);
+
+// Plane-tiled kernels for d = 2, 3 (grid_tile.hip).
+template <typename T> bool tile_kernels_fit(const Geom &ge);
+template <typename T> int64_t tile_slot_cells(const Geom &ge);
+template <typename T>
+void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
+                        hipStream_t s);
+template <typename T>
+void launch_tile_spread(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
+                        hipStream_t s, hipEvent_t mid);
 
 }  // namespace cunfft

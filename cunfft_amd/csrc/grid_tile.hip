@@ -1,0 +1,966 @@
+// grid_tile.hip -- plane-tiled gridding kernels for d = 2, 3 on gfx950.
+//
+// Forward gridding f_j = sum_l g_l phi(x_j - l/n) (src/cunfft_kernel.cu:514-1093)
+// and adjoint gridding g_l = sum_j f_j phi(x_j - l/n) (src/cunfft_kernel.cu:
+// 608-1197), one workgroup per chunk of the nodes binned to one grid tile
+// (bin.hip).  The window of a node covers W = 2m+2 cells per axis; its cells
+// in the padded tile held in LDS are visited one plane (leading-axis index) at
+// a time, with the lanes of a wavefront spread over the W rows x window width
+// of the plane:
+//   * each lane owns one 16-byte unit of a row: two FP32 complex cells, or one
+//     FP64 cell, so every LDS access is one ds_read_b128 / ds_write_b128;
+//   * one instruction holds the cells of a single node, so the lanes of an
+//     instruction never touch the same cell and the adjoint accumulates with
+//     plain read-add-write;
+//   * the lane -> (row, unit) assignment is a compile-time table that puts 16
+//     lanes on 16 distinct LDS bank groups for the chosen row pitch (gfx950
+//     serves 16 lanes of a 128-bit access per clock from 64 banks of 4 bytes).
+// Nodes are taken in batches: a prologue turns each node's descriptor into its
+// window origin and its three weight vectors in LDS (one thread per weight,
+// hardware exp2 for FP32, exp for FP64), and the gridding loop reads them back
+// with broadcast LDS reads while the next batch's descriptors are loaded.
+//
+// Adjoint: the accumulated padded tile of each chunk is written to a slot in
+// a destination-blocked order, and k_tile_fold sums, for every grid tile, the
+// slot blocks of the chunks whose padded tiles cover it: no atomics, no zero
+// fill, and a result that does not depend on scheduling.
+#include "bin.hpp"
+
+namespace cunfft {
+
+constexpr int kLdsBudget = 80 * 1024;   // a padded tile up to this size leaves room for 2 per CU
+constexpr int kLdsMax = 150 * 1024;
+
+// ------------------------------------------------- compile-time layout --
+template <int N> struct LaneTab {
+    uint16_t v[N];
+};
+
+// 16-byte LDS bank group (0..15) of item (q, r, u): plane q * PS, row r, unit u.
+constexpr int unit_group(int q, int r, int u, int PS, int PP16, int RP16)
+{
+    return (q * PS * PP16 + r * RP16 + u) & 15;
+}
+
+constexpr int max_group_load(int G, int R, int NU, int PS, int P1, int RP, int CB)
+{
+    int cnt[16] = {};
+    const int RP16 = RP * CB / 16, PP16 = P1 * RP * CB / 16;
+    for (int q = 0; q < G; ++q)
+        for (int r = 0; r < R; ++r)
+            for (int u = 0; u < NU; ++u) ++cnt[unit_group(q, r, u, PS, PP16, RP16)];
+    int mx = 0;
+    for (int k = 0; k < 16; ++k) mx = cnt[k] > mx ? cnt[k] : mx;
+    return mx;
+}
+
+// Row pitch (cells): the smallest one, from the needed row length RL up, whose
+// items spread over the NG 16-lane groups without bank conflicts, within the
+// LDS budget; otherwise the least conflicted one.
+constexpr int choose_pitch(int G, int R, int NU, int PS, int P0, int P1, int RL, int CB, int NG)
+{
+    const int step = 16 / CB;
+    const bool room = (long)P0 * P1 * RL * CB <= kLdsBudget;
+    int best = RL, best_cost = 1 << 30;
+    for (int RP = RL; RP < RL + 16; RP += step) {
+        if (room && (long)P0 * P1 * RP * CB > kLdsBudget) break;
+        if (!room && RP != RL) break;
+        const int c = max_group_load(G, R, NU, PS, P1, RP, CB);
+        const int cost = (c + NG - 1) / NG;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = RP;
+        }
+    }
+    return best;
+}
+
+// Lane table: items listed group-major by bank group, most loaded group
+// first, and dealt round-robin over the NG groups of 16 lanes.  A bank group
+// with at most NG items then never appears twice in one 16-lane group.  Within
+// a group, the item of bank group p sits in lane p: any shift of the node's
+// origin then keeps the 16 lanes on 16 bank groups (what 128-bit reads need)
+// and each half of 8 lanes on 8 distinct bank groups mod 8 (what 128-bit
+// writes need, measured on gfx950).
+// Entry: q << 10 | r << 5 | u, or 0xFFFF for an idle lane.
+template <int NT>
+constexpr LaneTab<NT> make_lanes(int G, int R, int NU, int PS, int P1, int RP, int CB, int NG)
+{
+    LaneTab<NT> t{};
+    for (int i = 0; i < NT; ++i) t.v[i] = 0xFFFF;
+    const int RP16 = RP * CB / 16, PP16 = P1 * RP * CB / 16;
+    int cnt[16] = {};
+    for (int q = 0; q < G; ++q)
+        for (int r = 0; r < R; ++r)
+            for (int u = 0; u < NU; ++u) ++cnt[unit_group(q, r, u, PS, PP16, RP16)];
+    int order[16] = {};
+    for (int k = 0; k < 16; ++k) order[k] = k;
+    for (int a = 0; a < 16; ++a)
+        for (int b = a + 1; b < 16; ++b)
+            if (cnt[order[b]] > cnt[order[a]]) {
+                const int tmp = order[a];
+                order[a] = order[b];
+                order[b] = tmp;
+            }
+    int k = 0;
+    for (int oi = 0; oi < 16; ++oi)
+        for (int q = 0; q < G; ++q)
+            for (int r = 0; r < R; ++r)
+                for (int u = 0; u < NU; ++u)
+                    if (unit_group(q, r, u, PS, PP16, RP16) == order[oi]) {
+                        // lane order[oi] of the next group that has it free; if a bank
+                        // group has more items than there are groups, any free lane
+                        const int p = order[oi];
+                        int slot = -1;
+                        for (int g = 0; g < NT / 16 && slot < 0; ++g) {
+                            const int grp = (k + g) % (NT / 16);
+                            if (t.v[grp * 16 + p] == 0xFFFF) slot = grp * 16 + p;
+                        }
+                        for (int l = 0; l < NT && slot < 0; ++l)
+                            if (t.v[l] == 0xFFFF) slot = l;
+                        t.v[slot] = (uint16_t)(q << 10 | r << 5 | u);
+                        ++k;
+                    }
+    return t;
+}
+
+// Layout of the padded tile in LDS and of one node's window over the lanes.
+// PS is the plane step between the planes one wavefront visits (the number of
+// wavefronts sharing the adjoint tile, 1 for the forward).
+template <typename T, int D, int W, int PS> struct TileLayout {
+    static constexpr int PSV = PS;
+    static constexpr int CB = (int)sizeof(cplx_t<T>);
+    static constexpr int CPL = 16 / CB;                      // cells per lane unit
+    static constexpr int T0 = TileShape<D>::T[0];
+    static constexpr int T1 = D == 3 ? TileShape<D>::T[1] : 1;
+    static constexpr int TL = TileShape<D>::T[D - 1];
+    static constexpr int P0 = T0 + W - 1;
+    static constexpr int P1 = D == 3 ? T1 + W - 1 : 1;
+    static constexpr int PL = TL + W - 1;
+    static constexpr int R = D == 3 ? W : 1;                 // window rows per plane
+    static constexpr int NU = CPL == 2 ? W / 2 + 1 : W;      // lane units per window row
+    static constexpr int RL = CPL == 2 ? 2 * ((TL - 1) / 2 + NU) : PL;  // row cells used
+    static constexpr int OWN = (W + PS - 1) / PS;            // planes per wavefront per node
+    static constexpr int GM = 64 / (R * NU) > 0 ? 64 / (R * NU) : 1;
+    static constexpr int G = GM < OWN ? GM : OWN;            // planes per lane group
+    static constexpr int NGRP = (OWN + G - 1) / G;
+    static constexpr int I = G * R * NU;
+    static constexpr int NG = (I + 15) / 16;
+    static constexpr int NI = (NG + 3) / 4;                  // LDS instructions per plane group
+    static constexpr int RP = choose_pitch(G, R, NU, PS, P0, P1, RL, CB, NG);
+    static constexpr int PP = P1 * RP;
+    static constexpr int CELLS = P0 * PP;
+    static constexpr int SC = P0 * P1 * PL;                  // adjoint slot cells
+    static constexpr int Q0 = (P0 + T0 - 1) / T0, Q1 = (P1 + T1 - 1) / T1,
+                         QL = (PL + TL - 1) / TL;            // slot blocks per axis
+    static constexpr LaneTab<NI * 64> lanes = make_lanes<NI * 64>(G, R, NU, PS, P1, RP, CB, NG);
+    static constexpr bool fits = (long)CELLS * CB <= kLdsMax && W <= 18 && NU <= 31 && G < 64;
+    // a plane group of some node can reach past its window (adjoint dummy units needed)
+    static constexpr bool NEED_DUMMY = (PS - 1) + PS * (NGRP * G - 1) >= W;
+    // planes loaded per batch: at most 12 outstanding 16-byte LDS reads
+    static constexpr int PB = 12 / NI > 0 ? (12 / NI < NGRP ? 12 / NI : NGRP) : 1;
+
+    // Slot block q_t along an axis with tile T_ and padded extent P_.
+    static constexpr int ext(int T_, int P_, int q) { return P_ - q * T_ < T_ ? P_ - q * T_ : T_; }
+    // Offset of block (q0, q1, qL) in the destination-blocked slot order.
+    static constexpr int block_off(int q0, int q1, int qL)
+    {
+        return q0 * T0 * P1 * PL + ext(T0, P0, q0) * (q1 * T1 * PL + ext(T1, P1, q1) * qL * TL);
+    }
+};
+
+// ----------------------------------------------------------- helpers ----
+template <typename T> struct WinConst {
+    T kb[3];    // FP32: log2(e) / b_t (for exp2); FP64: 1 / b_t (for exp)
+    T nrm[3];   // (pi b_t)^-1/2
+    int m;
+};
+
+template <typename T> __device__ __forceinline__ T wexp(T v);
+template <> __device__ __forceinline__ float wexp<float>(float v)
+{
+    return __builtin_amdgcn_exp2f(v);
+}
+template <> __device__ __forceinline__ double wexp<double>(double v) { return exp(v); }
+
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+template <typename T> using vec16_t = typename Vec16<T>::type;
+
+template <int CTRL, int ROWS> __device__ __forceinline__ float dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Sum of the 16 lanes of each row of the wavefront, in the row's last lane.
+template <typename T> __device__ __forceinline__ T row_sum15(T v)
+{
+    v += dpp<0x111, 0xf>(v);
+    v += dpp<0x112, 0xf>(v);
+    v += dpp<0x114, 0xf>(v);
+    v += dpp<0x118, 0xf>(v);
+    return v;
+}
+
+// Compiler-only barrier: keeps a wavefront's LDS accesses in program order
+// (the LDS executes the instructions of one wavefront in order).
+__device__ __forceinline__ void lds_order() { __asm__ volatile("" ::: "memory"); }
+
+// Per-lane item of the lane table: plane q * PS, row r, unit u of a node's
+// window, as a cell offset from the window origin in the padded tile.
+struct LaneItem {
+    int off;    // cells
+    int q;      // plane within the plane group (0 when a group holds one plane)
+    int r;      // window row
+    int u2;     // first column tap of the unit, before the parity shift
+    int ok;
+};
+
+template <typename L> __device__ __forceinline__ LaneItem lane_item(int i, int lane)
+{
+    constexpr LaneTab<L::NI * 64> tab = L::lanes;
+    const int e = tab.v[i * 64 + lane];
+    LaneItem it;
+    it.ok = e != 0xFFFF;
+    const int q = it.ok ? e >> 10 : 0, r = it.ok ? (e >> 5) & 31 : 0, u = it.ok ? e & 31 : 0;
+    it.q = L::G == 1 ? 0 : q;   // one plane per lane group: the plane index is uniform
+    it.r = r;
+    it.u2 = L::CPL * u;
+    it.off = q * L::PSV * L::PP + r * L::RP + L::CPL * u;
+    return it;
+}
+
+// Window weights of one batch of nodes, written to LDS by the wavefront that
+// grids the batch with one lane per weight, so the gridding loop evaluates no
+// exponentials: its lanes fetch their plane, row and column weights with
+// broadcast LDS reads issued together with the node's tile accesses.
+template <typename T, int D, int W, int NB> struct alignas(16) WinBatch {
+    static constexpr int V0 = (W + 3) & ~3;
+    T w0[NB][V0];         // plane weights
+    T w1[NB][W];          // row weights (d = 3)
+    T w2[NB][W + 4];      // column weight of tap l at l + 1; zero at 0 and past W
+    T fx[NB][D];          // c_t - m per axis (prologue input)
+};
+
+// Nodes per wavefront batch: the largest power of two up to 16 that keeps the
+// workgroups per CU the padded tile alone allows.
+template <typename T, int D, int W>
+constexpr int batch_nodes(int tile_bytes, int nw, int extra_per_node)
+{
+    const int total = 160 * 1024;
+    const int per_cu = total / tile_bytes > 0 ? total / tile_bytes : 1;
+    const int budget = total / per_cu - tile_bytes;
+    return nw * ((int)sizeof(WinBatch<T, D, W, 16>) + 16 * extra_per_node) <= budget  ? 16
+           : nw * ((int)sizeof(WinBatch<T, D, W, 8>) + 8 * extra_per_node) <= budget ? 8
+           : nw * ((int)sizeof(WinBatch<T, D, W, 4>) + 4 * extra_per_node) <= budget ? 4
+                                                                                     : 2;
+}
+
+// Window origin of a node, held by the lane that loaded it.
+struct NodeRec {
+    int nbase;   // cell of the window origin in the padded tile (plane shift excluded)
+    int lo0;
+    int par;     // FP32: parity of the column origin (16-byte unit alignment)
+};
+
+// Batch prologue of one wavefront: lane n < nb turns node n's descriptor into
+// its record and axis offsets, then every lane evaluates window weights.
+// Within one wavefront LDS accesses execute in order, so the compiler
+// barriers suffice between the phases.
+template <typename T, int D, int W, int NB, typename L>
+__device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB> &wb, int nb, int lane,
+                                                  const NodeDesc<T> &nd, const WinConst<T> &wc)
+{
+    NodeRec r{0, 0, 0};
+    if (lane < nb) {
+        const uint32_t loc = (uint32_t)nd.loc;
+        r.lo0 = loc & 255;
+        const int lo1 = D == 3 ? (loc >> 8) & 255 : 0;
+        const int loL = (loc >> (8 * (D - 1))) & 255;
+        r.par = L::CPL == 2 ? (loL & 1) : 0;
+        r.nbase = r.lo0 * L::PP + lo1 * L::RP + loL - r.par;
+#pragma unroll
+        for (int t = 0; t < D; ++t) wb.fx[lane][t] = nd.f[t];
+        wb.w2[lane][0] = (T)0;
+        wb.w2[lane][W + 1] = (T)0;
+        wb.w2[lane][W + 2] = (T)0;
+        wb.w2[lane][W + 3] = (T)0;
+    }
+    lds_order();
+    // axes at compile time (the per-axis constants stay scalar), lanes over
+    // (node, tap), fixed trip count so all offsets are read before the exps
+    constexpr int NPASS = (NB * W + 63) / 64;
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+        T sv[NPASS];
+        int nv[NPASS], lv[NPASS];
+#pragma unroll
+        for (int ps = 0; ps < NPASS; ++ps) {
+            int it = lane + 64 * ps;
+            it = it < nb * W ? it : 0;
+            nv[ps] = it / W;
+            lv[ps] = it - nv[ps] * W;
+            sv[ps] = wb.fx[nv[ps]][t] + (T)(wc.m - lv[ps]);
+        }
+#pragma unroll
+        for (int ps = 0; ps < NPASS; ++ps) {
+            if (lane + 64 * ps >= nb * W) break;
+            const T w = wc.nrm[t] * wexp<T>(-wc.kb[t] * sv[ps] * sv[ps]);
+            if (t == 0) wb.w0[nv[ps]][lv[ps]] = w;
+            else if (t == D - 1) wb.w2[nv[ps]][lv[ps] + 1] = w;
+            else wb.w1[nv[ps]][lv[ps]] = w;
+        }
+    }
+    lds_order();
+    return r;
+}
+
+__device__ __forceinline__ int rlane_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ float rlane_t(float v, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ double rlane_t(double v, int k)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// ------------------------------------------------------------ gather ----
+// NW wavefronts share the staged tile and take batches of the chunk's nodes
+// round-robin.  Each lane accumulates its unit over the node's planes and
+// weighs the sum with its row and column weights; the sixteen lanes of each
+// row are summed with DPP, and the four row sums of each node are added by
+// the node's lane after the batch.
+template <typename T, int D, int W, int NW>
+__global__ __launch_bounds__(NW * 64) void k_tile_gather(
+    const cplx_t<T> *__restrict__ g, const NodeDesc<T> *__restrict__ desc,
+    const uint32_t *__restrict__ perm, const Chunk *__restrict__ chunks,
+    const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f, Geom ge, BinGeom bg,
+    WinConst<T> wc)
+{
+    using L = TileLayout<T, D, W, 1>;
+    using V = vec16_t<T>;
+    using C = cplx_t<T>;
+    constexpr int CPL = L::CPL;
+    constexpr int NT = NW * 64;
+    constexpr int NR = L::RL / CPL;               // lane units staged per row
+    constexpr int PB = L::PB;
+    constexpr int NB = batch_nodes<T, D, W>(L::CELLS * L::CB, NW, 4 * (int)sizeof(C));
+    __shared__ __align__(16) C tile[L::CELLS];
+    __shared__ WinBatch<T, D, W, NB> wbs[NW];
+    __shared__ C red[NW][NB][4];
+    if (blockIdx.x >= n_chunks[0]) return;
+    const Chunk ch = chunks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nbat = (ch.count + NB - 1) / NB;
+
+    // this wavefront's first batch in flight while the tile is staged
+    NodeDesc<T> nd{};
+    uint32_t pm = 0;
+    {
+        const uint32_t s = (uint32_t)wave * NB + lane;
+        if (lane < NB && s < ch.count) {
+            nd = desc[ch.start + s];
+            pm = perm[ch.start + s];
+        }
+    }
+
+    // ---- stage the padded tile: rows of RL cells, 16 bytes per load
+    {
+        uint32_t rem = ch.tile, o[3] = {0, 0, 0};
+        for (int t = D - 1; t >= 0; --t) {
+            o[t] = (rem % (uint32_t)bg.nt[t]) << bg.logT[t];
+            rem /= (uint32_t)bg.nt[t];
+        }
+        const uint32_t m0 = (uint32_t)ge.n[0] - 1, mL = (uint32_t)ge.n[D - 1] - 1;
+        const uint32_t m1 = D == 3 ? (uint32_t)ge.n[1] - 1 : 0;
+        const uint32_t s1 = D == 3 ? (uint32_t)ge.n[2] : 0;
+        const uint32_t s0 = D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : (uint32_t)ge.n[1];
+        V *tv = reinterpret_cast<V *>(tile);
+        const V *gv = reinterpret_cast<const V *>(g);
+        for (int idx = tid; idx < L::P0 * L::P1 * NR; idx += NT) {
+            const int row = idx / NR, un = idx - row * NR;
+            const int p0 = row / L::P1, p1 = row - p0 * L::P1;
+            uint32_t cell = ((o[0] + p0) & m0) * s0 + ((o[D - 1] + CPL * un) & mL);
+            if (D == 3) cell += ((o[1] + p1) & m1) * s1;
+            tv[(row * L::RP) / CPL + un] = gv[cell / CPL];
+        }
+    }
+    __syncthreads();
+
+    LaneItem it[L::NI];
+#pragma unroll
+    for (int i = 0; i < L::NI; ++i) it[i] = lane_item<L>(i, lane);
+    const char *tb = reinterpret_cast<const char *>(tile);
+    WinBatch<T, D, W, NB> &wb = wbs[wave];
+
+    for (uint32_t bi = wave; bi < nbat; bi += NW) {
+        const uint32_t s0 = bi * NB;
+        const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
+        const NodeRec rc = batch_prologue<T, D, W, NB, L>(wb, nb, lane, nd, wc);
+        const uint32_t pm_out = pm;
+        {
+            const uint32_t s = s0 + NW * NB + lane;
+            if (lane < NB && s < ch.count) {
+                nd = desc[ch.start + s];
+                pm = perm[ch.start + s];
+            }
+        }
+        for (int k = 0; k < nb; ++k) {
+            const int nbase = rlane_i(rc.nbase, k), par = rlane_i(rc.par, k);
+            T a0x[L::NI], a0y[L::NI], a1x[L::NI], a1y[L::NI];
+#pragma unroll
+            for (int i = 0; i < L::NI; ++i) a0x[i] = a0y[i] = a1x[i] = a1y[i] = 0;
+            T part_x = 0, part_y = 0;
+#pragma unroll
+            for (int i = 0; i < L::NI; ++i) {
+                if (!it[i].ok) continue;
+                const uint32_t lb = (uint32_t)(nbase + it[i].off - it[i].q * L::PP) * L::CB;
+                const int cw = it[i].u2 - par + 1;
+                T w1v = 0, w2a = 0, w2b = 0;
+#pragma unroll
+                for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
+                    V v[PB];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int gq = g0 + j;
+                        if (gq >= L::NGRP) break;
+                        int a = gq * L::G + it[i].q;
+                        a = a < W ? a : W - 1;   // planes past the window: weight 0 below
+                        v[j] = *reinterpret_cast<const V *>(tb + lb + (uint32_t)(a * L::PP * L::CB));
+                    }
+                    // the node's weights are read only after its tile reads are
+                    // issued, and all of them before any use: one LDS round trip
+                    lds_order();
+                    if (g0 == 0) {
+                        w1v = D == 3 ? wb.w1[k][it[i].r] : (T)1;
+                        w2a = wb.w2[k][cw];
+                        w2b = CPL == 2 ? wb.w2[k][cw + 1] : (T)0;
+                    }
+                    T w0r[PB];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int a = (g0 + j) * L::G + it[i].q;
+                        w0r[j] = wb.w0[k][a < W ? a : W - 1];
+                    }
+                    lds_order();
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int gq = g0 + j;
+                        if (gq >= L::NGRP) break;
+                        const int a = gq * L::G + it[i].q;
+                        T w0a = w0r[j];
+                        if constexpr (L::G > 1) w0a = a < W ? w0a : (T)0;
+                        if constexpr (CPL == 2) {
+                            a0x[i] += w0a * v[j].x;
+                            a0y[i] += w0a * v[j].y;
+                            a1x[i] += w0a * v[j].z;
+                            a1y[i] += w0a * v[j].w;
+                        } else {
+                            a0x[i] += w0a * v[j].x;
+                            a0y[i] += w0a * v[j].y;
+                        }
+                    }
+                }
+                const T wu = w1v * w2a;
+                part_x += wu * a0x[i];
+                part_y += wu * a0y[i];
+                if constexpr (CPL == 2) {
+                    const T wv = w1v * w2b;
+                    part_x += wv * a1x[i];
+                    part_y += wv * a1y[i];
+                }
+            }
+            part_x = row_sum15<T>(part_x);
+            part_y = row_sum15<T>(part_y);
+            if ((lane & 15) == 15) red[wave][k][lane >> 4] = cmake<T>(part_x, part_y);
+        }
+        lds_order();
+        if (lane < nb) {
+            const C r0 = red[wave][lane][0], r1 = red[wave][lane][1];
+            const C r2 = red[wave][lane][2], r3 = red[wave][lane][3];
+            f[pm_out] = cmake<T>((r0.x + r1.x) + (r2.x + r3.x), (r0.y + r1.y) + (r2.y + r3.y));
+        }
+        lds_order();
+    }
+}
+
+// ------------------------------------------------------------ spread ----
+// NW wavefronts share the tile; wavefront w owns the planes p0 = w (mod NW),
+// so wavefronts never touch the same cell, and every wavefront walks all the
+// chunk's nodes in order (each wavefront keeps its own weight batch).  The
+// cells of one node are disjoint, so a node's reads are all issued before its
+// adds and writes; consecutive nodes of one wavefront may overlap, so their
+// read-add-write sequences stay in program order.  When a plane group can
+// reach past the window, the lanes concerned read and write a private dummy
+// unit past the tile.  fs holds the coefficients in sorted node order.
+template <typename T, int D, int W, int NW>
+__global__ __launch_bounds__(NW * 64) void k_tile_spread(
+    const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
+    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks,
+    cplx_t<T> *__restrict__ slots, WinConst<T> wc)
+{
+    using L = TileLayout<T, D, W, NW>;
+    using V = vec16_t<T>;
+    using C = cplx_t<T>;
+    constexpr int CPL = L::CPL;
+    constexpr int NT = NW * 64;
+    constexpr int PB = L::PB;
+    constexpr bool DUMMY = L::NEED_DUMMY;
+    constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
+    constexpr int NB = batch_nodes<T, D, W>(TILE_UNITS * 16, NW, 0);
+    __shared__ __align__(16) V tile[TILE_UNITS];
+    __shared__ WinBatch<T, D, W, NB> wbs[NW];
+    if (blockIdx.x >= n_chunks[0]) return;
+    const Chunk ch = chunks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    NodeDesc<T> nd{};
+    C fv = cmake<T>(0, 0);
+    if (lane < NB && (uint32_t)lane < ch.count) {
+        nd = desc[ch.start + lane];
+        fv = fs[ch.start + lane];
+    }
+    {
+        V z;
+        if constexpr (CPL == 2) z = make_float4(0.f, 0.f, 0.f, 0.f);
+        else z = make_double2(0.0, 0.0);
+        for (int idx = tid; idx < TILE_UNITS; idx += NT) tile[idx] = z;
+    }
+    __syncthreads();
+
+    LaneItem it[L::NI];
+#pragma unroll
+    for (int i = 0; i < L::NI; ++i) it[i] = lane_item<L>(i, lane);
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t dummy = (uint32_t)(L::CELLS / CPL + lane) * 16;
+    WinBatch<T, D, W, NB> &wb = wbs[wave];
+
+    for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
+        const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
+        const NodeRec rc = batch_prologue<T, D, W, NB, L>(wb, nb, lane, nd, wc);
+        const C fc = fv;
+        {
+            const uint32_t s = s0 + NB + lane;
+            if (lane < NB && s < ch.count) {
+                nd = desc[ch.start + s];
+                fv = fs[ch.start + s];
+            }
+        }
+        for (int k = 0; k < nb; ++k) {
+            const int lo0 = rlane_i(rc.lo0, k), par = rlane_i(rc.par, k);
+            const int a0 = NW == 1 ? 0 : ((wave - lo0) & (NW - 1));
+            const int nbase = rlane_i(rc.nbase, k) + a0 * L::PP;
+            const T fr = rlane_t(fc.x, k), fi = rlane_t(fc.y, k);
+#pragma unroll
+            for (int i = 0; i < L::NI; ++i) {
+                if (!it[i].ok) continue;
+                const uint32_t lb = (uint32_t)(nbase + it[i].off) * L::CB;
+                const int cw = it[i].u2 - par + 1;
+#pragma unroll
+                for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
+                    V v[PB];
+                    uint32_t ad[PB];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int gq = g0 + j;
+                        if (gq >= L::NGRP) break;
+                        ad[j] = lb + (uint32_t)(NW * gq * L::G * L::PP * L::CB);
+                        if constexpr (DUMMY) {
+                            if ((NW - 1) + NW * (gq * L::G + L::G - 1) >= W) {
+                                const int a = a0 + NW * (gq * L::G + it[i].q);
+                                ad[j] = a < W ? ad[j] : dummy;
+                            }
+                        }
+                        v[j] = *reinterpret_cast<const V *>(tb + ad[j]);
+                    }
+                    // the node's weights are read only after its tile reads are
+                    // issued, and all of them before any use: one LDS round trip
+                    lds_order();
+                    const T w1v = D == 3 ? wb.w1[k][it[i].r] : (T)1;
+                    const T w2a = wb.w2[k][cw];
+                    const T w2b = CPL == 2 ? wb.w2[k][cw + 1] : (T)0;
+                    T w0r[PB];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int a = a0 + NW * ((g0 + j) * L::G + it[i].q);
+                        w0r[j] = wb.w0[k][a < W ? a : W - 1];
+                    }
+                    lds_order();
+                    const T wu = w1v * w2a;
+                    const T fux = fr * wu, fuy = fi * wu;
+                    T fvx = 0, fvy = 0;
+                    if constexpr (CPL == 2) {
+                        const T wv = w1v * w2b;
+                        fvx = fr * wv;
+                        fvy = fi * wv;
+                    }
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int gq = g0 + j;
+                        if (gq >= L::NGRP) break;
+                        const T w0a = w0r[j];
+                        if constexpr (CPL == 2) {
+                            v[j].x += w0a * fux;
+                            v[j].y += w0a * fuy;
+                            v[j].z += w0a * fvx;
+                            v[j].w += w0a * fvy;
+                        } else {
+                            v[j].x += w0a * fux;
+                            v[j].y += w0a * fuy;
+                        }
+                        *reinterpret_cast<V *>(tb + ad[j]) = v[j];
+                    }
+                }
+            }
+            lds_order();
+        }
+    }
+    __syncthreads();
+
+    // ---- padded tile -> slot, destination-blocked
+    const C *tc = reinterpret_cast<const C *>(tile);
+    C *slot = slots + (int64_t)blockIdx.x * L::SC;
+#pragma unroll
+    for (int q0 = 0; q0 < L::Q0; ++q0)
+#pragma unroll
+        for (int q1 = 0; q1 < L::Q1; ++q1)
+#pragma unroll
+            for (int qL = 0; qL < L::QL; ++qL) {
+                const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1),
+                          EL = L::ext(L::TL, L::PL, qL);
+                const int n = E0 * E1 * EL;
+                C *dst = slot + L::block_off(q0, q1, qL);
+                for (int c0 = 0; c0 < n; c0 += 8 * NT) {
+                    C v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        int c = c0 + tid + u * NT;
+                        c = c < n ? c : n - 1;
+                        const int r0 = c / (E1 * EL), rr = c - r0 * (E1 * EL);
+                        const int r1 = rr / EL, rL = rr - r1 * EL;
+                        v[u] = tc[(q0 * L::T0 + r0) * L::PP + (q1 * L::T1 + r1) * L::RP +
+                                  qL * L::TL + rL];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int c = c0 + tid + u * NT;
+                        if (c < n) dst[c] = v[u];
+                    }
+                }
+            }
+}
+
+// Coefficients in sorted node order for the adjoint: fs[s] = f[perm[s]].
+template <typename T>
+__global__ __launch_bounds__(256) void k_sorted_coeffs(const cplx_t<T> *__restrict__ f,
+                                                       const uint32_t *__restrict__ perm,
+                                                       cplx_t<T> *__restrict__ fs, int64_t M)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s < M) fs[s] = f[perm[s]];
+}
+
+// -------------------------------------------------------------- fold ----
+// g on one grid tile = sum over the chunks of the tiles whose padded region
+// covers it of the matching slot block.  One workgroup per tile; each block a
+// thread reads is contiguous over the workgroup.  The first chunk of the
+// candidate tiles of one leading-axis offset is loaded at once (a dense tile's
+// further chunks follow), and the sums run in a fixed order: the result does
+// not depend on scheduling.
+template <typename T, int D, int W, int PS>
+__global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__ slots,
+                                                   const uint32_t *__restrict__ tile_count,
+                                                   const uint32_t *__restrict__ chunk_start,
+                                                   cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
+{
+    using L = TileLayout<T, D, W, PS>;
+    using C = cplx_t<T>;
+    constexpr int CORE = L::T0 * L::T1 * L::TL;
+    constexpr int CPT = (CORE + 255) / 256;
+    constexpr int QR = L::Q1 * L::QL;            // candidates per leading-axis offset
+    uint32_t tk[3] = {0, 0, 0};
+    {
+        uint32_t rem = blockIdx.x;
+        for (int t = D - 1; t >= 0; --t) {
+            tk[t] = rem % (uint32_t)bg.nt[t];
+            rem /= (uint32_t)bg.nt[t];
+        }
+    }
+    int r0[CPT], r1[CPT], rL[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int c = threadIdx.x + 256 * k;
+        r0[k] = c / (L::T1 * L::TL);
+        r1[k] = (c / L::TL) % L::T1;
+        rL[k] = c % L::TL;
+    }
+    T sx[CPT], sy[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) sx[k] = sy[k] = 0;
+#pragma unroll
+    for (int q0 = 0; q0 < L::Q0; ++q0) {
+        uint32_t c0[QR], nch[QR];
+        int li[QR][CPT];
+        bool ok[QR][CPT];
+        C v[QR][CPT];
+#pragma unroll
+        for (int qq = 0; qq < QR; ++qq) {
+            const int q1 = qq / L::QL, qL = qq % L::QL;
+            const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1),
+                      EL = L::ext(L::TL, L::PL, qL);
+            const int qv[3] = {q0, D == 3 ? q1 : qL, qL};
+            uint32_t src = 0;
+            for (int t = 0; t < D; ++t) {
+                const uint32_t ntt = (uint32_t)bg.nt[t];
+                src = src * ntt + (tk[t] + ntt * 4 - (uint32_t)qv[t]) % ntt;
+            }
+            const uint32_t cnt = tile_count[src];
+            c0[qq] = chunk_start[src];
+            nch[qq] = (cnt + kChunk - 1) / kChunk;
+            const int boff = L::block_off(q0, q1, qL);
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                ok[qq][k] = cnt > 0 && threadIdx.x + 256 * k < CORE && r0[k] < E0 && r1[k] < E1 &&
+                            rL[k] < EL;
+                li[qq][k] = boff + (r0[k] * E1 + r1[k]) * EL + rL[k];
+            }
+        }
+#pragma unroll
+        for (int qq = 0; qq < QR; ++qq)
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                // a candidate with no nodes reads chunk 0 and adds nothing
+                const C *sp = slots + (int64_t)(ok[qq][k] ? c0[qq] : 0) * L::SC;
+                v[qq][k] = sp[ok[qq][k] ? li[qq][k] : 0];
+            }
+#pragma unroll
+        for (int qq = 0; qq < QR; ++qq)
+#pragma unroll
+            for (int k = 0; k < CPT; ++k)
+                if (ok[qq][k]) {
+                    sx[k] += v[qq][k].x;
+                    sy[k] += v[qq][k].y;
+                }
+        // further chunks of dense candidate tiles, four loads in flight
+#pragma unroll
+        for (int qq = 0; qq < QR; ++qq)
+            for (uint32_t c = 1; c < nch[qq]; c += 4) {
+                C w[4][CPT];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool in = c + u < nch[qq];
+                    const C *sp = slots + (int64_t)(in ? c0[qq] + c + u : 0) * L::SC;
+#pragma unroll
+                    for (int k = 0; k < CPT; ++k) w[u][k] = sp[ok[qq][k] && in ? li[qq][k] : 0];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int k = 0; k < CPT; ++k)
+                        if (ok[qq][k] && c + u < nch[qq]) {
+                            sx[k] += w[u][k].x;
+                            sy[k] += w[u][k].y;
+                        }
+            }
+    }
+    const uint32_t nL = (uint32_t)ge.n[D - 1];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+        if (threadIdx.x + 256 * k < CORE) {
+            uint32_t cell = (tk[0] * L::T0 + r0[k]) * (D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : nL);
+            if (D == 3) cell += (tk[1] * L::T1 + r1[k]) * nL;
+            cell += tk[D - 1] * L::TL + rL[k];
+            g[cell] = cmake<T>(sx[k], sy[k]);
+        }
+}
+
+// ------------------------------------------------------------ dispatch --
+constexpr int kGatherWaves = 4;
+// Wavefronts sharing one adjoint tile, each owning every other plane: the
+// FP32 tile leaves room for three such workgroups per CU, the FP64 tile for
+// two (measured faster than one wavefront per tile, and than four).
+template <typename T> struct SpreadWaves {
+    static constexpr int value = 2;
+};
+
+template <typename T> static WinConst<T> win_const(const Geom &ge)
+{
+    WinConst<T> w{};
+    const double sc = sizeof(T) == 4 ? 1.4426950408889634 : 1.0;   // log2(e) for exp2
+    for (int t = 0; t < 3; ++t) {
+        w.kb[t] = (T)(ge.invb[t] * sc);
+        w.nrm[t] = (T)ge.nrm[t];
+    }
+    w.m = ge.m;
+    return w;
+}
+
+template <typename T, int D, int W> static constexpr bool fits_all()
+{
+    return TileLayout<T, D, W, 1>::fits && TileLayout<T, D, W, SpreadWaves<T>::value>::fits;
+}
+
+#define CUNFFT_TILE_W(CALL)                                    \
+    switch (ge.W) {                                            \
+    case 4: CALL(4); break;                                    \
+    case 6: CALL(6); break;                                    \
+    case 8: CALL(8); break;                                    \
+    case 10: CALL(10); break;                                  \
+    case 12: CALL(12); break;                                  \
+    case 14: CALL(14); break;                                  \
+    case 16: CALL(16); break;                                  \
+    case 18: CALL(18); break;                                  \
+    default: return false;                                     \
+    }
+
+template <typename T, int D, int W> static bool fits_dw() { return fits_all<T, D, W>(); }
+
+template <typename T> bool tile_kernels_fit(const Geom &ge)
+{
+    if (ge.d == 2) {
+#define FIT2(w) return fits_dw<T, 2, w>()
+        CUNFFT_TILE_W(FIT2)
+#undef FIT2
+    } else if (ge.d == 3) {
+#define FIT3(w) return fits_dw<T, 3, w>()
+        CUNFFT_TILE_W(FIT3)
+#undef FIT3
+    }
+    return false;
+}
+
+template <typename T, int D, int W> static int64_t slot_cells_dw()
+{
+    return TileLayout<T, D, W, SpreadWaves<T>::value>::SC;
+}
+
+template <typename T> int64_t tile_slot_cells(const Geom &ge)
+{
+    auto run = [&]() -> int64_t {
+        if (ge.d == 2) {
+            switch (ge.W) {
+            case 4: return slot_cells_dw<T, 2, 4>();
+            case 6: return slot_cells_dw<T, 2, 6>();
+            case 8: return slot_cells_dw<T, 2, 8>();
+            case 10: return slot_cells_dw<T, 2, 10>();
+            case 12: return slot_cells_dw<T, 2, 12>();
+            case 14: return slot_cells_dw<T, 2, 14>();
+            case 16: return slot_cells_dw<T, 2, 16>();
+            case 18: return slot_cells_dw<T, 2, 18>();
+            }
+        } else {
+            switch (ge.W) {
+            case 4: return slot_cells_dw<T, 3, 4>();
+            case 6: return slot_cells_dw<T, 3, 6>();
+            case 8: return slot_cells_dw<T, 3, 8>();
+            case 10: return slot_cells_dw<T, 3, 10>();
+            case 12: return slot_cells_dw<T, 3, 12>();
+            case 14: return slot_cells_dw<T, 3, 14>();
+            case 16: return slot_cells_dw<T, 3, 16>();
+            case 18: return slot_cells_dw<T, 3, 18>();
+            }
+        }
+        return 0;
+    };
+    return run();
+}
+
+template <typename T, int D, int W>
+static void tile_gather_dw(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
+                           hipStream_t s)
+{
+    if constexpr (fits_all<T, D, W>()) {
+        k_tile_gather<T, D, W, kGatherWaves><<<(unsigned)b.max_chunks, kGatherWaves * 64, 0, s>>>(
+            g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, f, ge,
+            bin_geom(b), win_const<T>(ge));
+    } else {
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
+    }
+}
+
+template <typename T, int D, int W>
+static void tile_spread_dw(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
+                           hipStream_t s, hipEvent_t mid)
+{
+    if constexpr (fits_all<T, D, W>()) {
+        cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
+        cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
+        k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
+        constexpr int NWS = SpreadWaves<T>::value;
+        k_tile_spread<T, D, W, NWS><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
+            static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
+            win_const<T>(ge));
+        if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
+        k_tile_fold<T, D, W, SpreadWaves<T>::value><<<(unsigned)b.ntiles, 256, 0, s>>>(
+            slots, b.tile_count, b.chunk_start, g, ge, bin_geom(b));
+    } else {
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
+    }
+}
+
+template <typename T>
+void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
+                        hipStream_t s)
+{
+    auto run = [&]() -> bool {
+        if (ge.d == 2) {
+#define G2(w) tile_gather_dw<T, 2, w>(ge, b, g, f, s)
+            CUNFFT_TILE_W(G2)
+#undef G2
+        } else {
+#define G3(w) tile_gather_dw<T, 3, w>(ge, b, g, f, s)
+            CUNFFT_TILE_W(G3)
+#undef G3
+        }
+        return true;
+    };
+    if (!run()) throw Error(CUNFFT_ERR_UNSUPPORTED, "tiled gridding needs m <= 8");
+    CUNFFT_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_tile_spread(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
+                        hipStream_t s, hipEvent_t mid)
+{
+    auto run = [&]() -> bool {
+        if (ge.d == 2) {
+#define S2(w) tile_spread_dw<T, 2, w>(ge, b, f, g, s, mid)
+            CUNFFT_TILE_W(S2)
+#undef S2
+        } else {
+#define S3(w) tile_spread_dw<T, 3, w>(ge, b, f, g, s, mid)
+            CUNFFT_TILE_W(S3)
+#undef S3
+        }
+        return true;
+    };
+    if (!run()) throw Error(CUNFFT_ERR_UNSUPPORTED, "tiled gridding needs m <= 8");
+    CUNFFT_HIP(hipGetLastError());
+}
+
+template bool tile_kernels_fit<float>(const Geom &);
+template bool tile_kernels_fit<double>(const Geom &);
+template int64_t tile_slot_cells<float>(const Geom &);
+template int64_t tile_slot_cells<double>(const Geom &);
+template void launch_tile_gather<float>(const Geom &, const Binning &, const float2 *, float2 *,
+                                        hipStream_t);
+template void launch_tile_gather<double>(const Geom &, const Binning &, const double2 *,
+                                         double2 *, hipStream_t);
+template void launch_tile_spread<float>(const Geom &, const Binning &, const float2 *, float2 *,
+                                        hipStream_t, hipEvent_t);
+template void launch_tile_spread<double>(const Geom &, const Binning &, const double2 *,
+                                         double2 *, hipStream_t, hipEvent_t);
+
+}  // namespace cunfft

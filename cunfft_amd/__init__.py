@@ -138,6 +138,9 @@ def lib():
     for s in ("cunfft_grad_x_dev", "cunfftf_grad_x_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, vp, vp]
         getattr(L, s).restype = ctypes.c_int
+    for s in ("cunfft_backward_dev", "cunfftf_backward_dev"):
+        getattr(L, s).argtypes = [PP, vp, vp, vp, vp, vp]
+        getattr(L, s).restype = ctypes.c_int
     for s in ("cundft_transform_rows_dev", "cundft_adjoint_cols_dev",
               "cunfftf_ndft_transform_rows_dev", "cunfftf_ndft_adjoint_cols_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, ctypes.c_longlong, vp, vp]
@@ -371,6 +374,13 @@ class Plan:
     def grad_x_dev(self, fhat_ptr, v_ptr, gx_ptr, stream=None):
         self._dev("cunfft_grad_x_dev", "cunfftf_grad_x_dev", ctypes.c_void_p(fhat_ptr),
                   ctypes.c_void_p(v_ptr), ctypes.c_void_p(gx_ptr), ctypes.c_void_p(stream))
+
+    def backward_dev(self, fhat_ptr, grad_f_ptr, grad_fhat_ptr, grad_x_ptr, stream=None):
+        """grad_fhat = A^H grad_f and grad_x = Re(conj(grad_f) dA/dx fhat); a 0 pointer
+        skips that output (cunfft_backward_dev)."""
+        self._dev("cunfft_backward_dev", "cunfftf_backward_dev", ctypes.c_void_p(fhat_ptr),
+                  ctypes.c_void_p(grad_f_ptr), ctypes.c_void_p(grad_fhat_ptr),
+                  ctypes.c_void_p(grad_x_ptr), ctypes.c_void_p(stream))
 
     def ndft_rows_dev(self, fhat_ptr, rows_ptr, n_rows, out_ptr, stream=None):
         self._dev("cundft_transform_rows_dev", "cunfftf_ndft_transform_rows_dev",

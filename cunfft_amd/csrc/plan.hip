@@ -660,9 +660,13 @@ template <typename T> void api_reinit(typename Traits<T>::plan *p, bool adjoint)
     });
 }
 
-hipStream_t pick_stream(void *stream, hipStream_t own)
+// Device-pointer entry points run on the caller's stream; NULL is the HIP
+// default (legacy null) stream, as for HIP's own calls -- the stream torch
+// reports as 0.  The plan's own stream serves only the host-buffer calls,
+// which synchronize it before they return.
+hipStream_t pick_stream(void *stream, hipStream_t /*own*/)
 {
-    return stream ? static_cast<hipStream_t>(stream) : own;
+    return static_cast<hipStream_t>(stream);
 }
 
 template <typename T>
@@ -716,6 +720,35 @@ int api_grad_x_dev(typename Traits<T>::plan *p, const void *fhat, const void *v,
         fft_passes<T>(p, im, fft_tables<T>(p, true), true, static_cast<const cplx_t<T> *>(fhat),
                       nullptr, g, s);
         launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(v), gx, s);
+    });
+}
+
+// Backward of f = A(x) fhat for an upstream gradient gf (complex gradient in
+// the conjugate-Wirtinger convention of torch): gfhat = A^H gf, and
+// gx[j*d+t] = Re(conj(gf_j) * d f_j / d x_jt).  Either output may be NULL.
+// The x-gradient reads the forward grid of fhat, so it runs first; the
+// adjoint then reuses the grid buffer.
+template <typename T>
+int api_backward_dev(typename Traits<T>::plan *p, const void *fhat, const void *gf, void *gfhat,
+                     T *gx, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!gf) throw Error(CUNFFT_ERR_INVALID_ARG, "upstream gradient is NULL");
+        if (gx && !fhat) throw Error(CUNFFT_ERR_INVALID_ARG, "the x-gradient needs fhat");
+        hipStream_t s = pick_stream(stream, im->stream);
+        if (gx) {
+            cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
+            fft_passes<T>(p, im, fft_tables<T>(p, true), true,
+                          static_cast<const cplx_t<T> *>(fhat), nullptr, g, s);
+            launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(gf), gx, s);
+        }
+        if (gfhat) {
+            double a, b, c;
+            run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(gf),
+                           static_cast<cplx_t<T> *>(gfhat), s, false, &a, &b, &c);
+        }
     });
 }
 
@@ -902,6 +935,13 @@ int cunfft_grad_x_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const cun
 {
     return api_grad_x_dev<double>(p, fhat, v, gx, s);
 }
+
+int cunfft_backward_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat,
+                        const cunfftDoubleComplex *grad_f, cunfftDoubleComplex *grad_fhat,
+                        double *grad_x, void *s)
+{
+    return api_backward_dev<double>(p, fhat, grad_f, grad_fhat, grad_x, s);
+}
 int cundft_transform_rows_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const long long *rows,
                               long long n_rows, cunfftDoubleComplex *out, void *s)
 {
@@ -999,6 +1039,13 @@ int cunfftf_grad_x_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat, const cu
                        float *gx, void *s)
 {
     return api_grad_x_dev<float>(p, fhat, v, gx, s);
+}
+
+int cunfftf_backward_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat,
+                         const cunfftFloatComplex *grad_f, cunfftFloatComplex *grad_fhat,
+                         float *grad_x, void *s)
+{
+    return api_backward_dev<float>(p, fhat, grad_f, grad_fhat, grad_x, s);
 }
 int cunfftf_ndft_transform_rows_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat,
                                     const long long *rows, long long n_rows, cunfftFloatComplex *out,

@@ -219,9 +219,10 @@ const char *cunfft_error_string(int status);
 const char *cunfft_last_message(void);
 
 /* Device-pointer entry points: operate on caller-owned device buffers on the
- * caller's HIP stream (stream may be NULL for the plan's own stream).  They
- * allocate nothing and return the status.  Buffers: fhat N_total complex,
- * f M_total complex, x d*M_total doubles. */
+ * caller's HIP stream (NULL: the HIP default stream).  They return the
+ * status; the node binning's buffers are allocated by the first transform of
+ * a plan, nothing afterwards.  Buffers: fhat N_total complex, f M_total
+ * complex, x d*M_total doubles. */
 /* Replace the plan's nodes (device copy) and rebuild the node binning. */
 int cunfft_set_nodes_dev(cunfft_plan *plan, const double *x_dev, void *stream);
 /* f = A fhat (forward NFFT, src/cunfft.cu:196-207). */
@@ -235,6 +236,14 @@ int cunfft_adjoint_dev(cunfft_plan *plan, const cunfftDoubleComplex *f_dev,
  * gather over the forward grid of fhat (weights recomputed, not stored). */
 int cunfft_grad_x_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
                       const cunfftDoubleComplex *v_dev, double *gx_dev, void *stream);
+/* Backward of f = A(x) fhat (no counterpart in the reference; the gradient
+ * entry point of a differentiable NFFT layer).  For an upstream gradient
+ * grad_f in the conjugate-Wirtinger convention (for L = 0.5||f - y||^2 it is
+ * f - y): grad_fhat = A^H grad_f (the adjoint NFFT) and grad_x[j*d+t] =
+ * Re(conj(grad_f_j) * d f_j / d x_jt).  Either output may be NULL. */
+int cunfft_backward_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
+                        const cunfftDoubleComplex *grad_f_dev, cunfftDoubleComplex *grad_fhat_dev,
+                        double *grad_x_dev, void *stream);
 /* Direct NDFT on the GPU in FP64 for the listed outputs only (rows of f or
  * entries of fhat by flat index); rows == NULL means all outputs. */
 int cundft_transform_rows_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
@@ -279,6 +288,9 @@ int cunfftf_adjoint_dev(cunfftf_plan *plan, const cunfftFloatComplex *f_dev,
                         cunfftFloatComplex *fhat_dev, void *stream);
 int cunfftf_grad_x_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                        const cunfftFloatComplex *v_dev, float *gx_dev, void *stream);
+int cunfftf_backward_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
+                         const cunfftFloatComplex *grad_f_dev, cunfftFloatComplex *grad_fhat_dev,
+                         float *grad_x_dev, void *stream);
 int cunfftf_ndft_transform_rows_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                                     const long long *rows_dev, long long n_rows,
                                     cunfftFloatComplex *out_dev, void *stream);
@@ -320,6 +332,7 @@ typedef float dTyp;
 #define cunfft_transform_dev cunfftf_transform_dev
 #define cunfft_adjoint_dev cunfftf_adjoint_dev
 #define cunfft_grad_x_dev cunfftf_grad_x_dev
+#define cunfft_backward_dev cunfftf_backward_dev
 #define cundft_transform_rows_dev cunfftf_ndft_transform_rows_dev
 #define cundft_adjoint_cols_dev cunfftf_ndft_adjoint_cols_dev
 #else

@@ -37,6 +37,10 @@ ATOMIC_SCATTER = 1 << 10
 PLAIN_GATHER = 1 << 11
 NODE_LANES = 1 << 12
 DEFAULT_CUT_OFF = 6
+TILE_SHAPES = 3                     # plane-tiled configurations 0 .. TILE_SHAPES-1
+TUNE_CONFIGS = TILE_SHAPES + 1      # + the node-lane configuration
+TUNE_FORWARD = 1
+TUNE_ADJOINT = 2
 
 _STATUS = {0: "success", 1: "invalid argument", 2: "unsupported parameters",
            3: "allocation failed", 4: "HIP runtime error", 5: "invalid plan state"}
@@ -144,6 +148,15 @@ def lib():
     for s in ("cundft_transform_rows_dev", "cundft_adjoint_cols_dev",
               "cunfftf_ndft_transform_rows_dev", "cunfftf_ndft_adjoint_cols_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, ctypes.c_longlong, vp, vp]
+        getattr(L, s).restype = ctypes.c_int
+    for s in ("cunfft_autotune", "cunfftf_autotune"):
+        getattr(L, s).argtypes = [PP, ctypes.c_int, ctypes.c_int, vp, vp]
+        getattr(L, s).restype = ctypes.c_int
+    for s in ("cunfft_get_config", "cunfftf_get_config"):
+        getattr(L, s).argtypes = [PP]
+        getattr(L, s).restype = ctypes.c_int
+    for s in ("cunfft_set_config", "cunfftf_set_config"):
+        getattr(L, s).argtypes = [PP, ctypes.c_int]
         getattr(L, s).restype = ctypes.c_int
     L.cunfft_last_message.restype = ctypes.c_char_p
     L.cunfft_error_string.restype = ctypes.c_char_p
@@ -381,6 +394,26 @@ class Plan:
         self._dev("cunfft_backward_dev", "cunfftf_backward_dev", ctypes.c_void_p(fhat_ptr),
                   ctypes.c_void_p(grad_f_ptr), ctypes.c_void_p(grad_fhat_ptr),
                   ctypes.c_void_p(grad_x_ptr), ctypes.c_void_p(stream))
+
+    # ------------------------------------------------------------ autotuner
+    def autotune(self, directions=TUNE_FORWARD | TUNE_ADJOINT, reps=5, stream=None):
+        """Time the gridding configurations on the plan's current nodes and keep
+        the fastest (cunfft_autotune).  Returns (chosen config, times in ms per
+        config, -1 where a configuration does not apply)."""
+        ms = (ctypes.c_float * TUNE_CONFIGS)()
+        self._dev("cunfft_autotune", "cunfftf_autotune", ctypes.c_int(directions),
+                  ctypes.c_int(reps), ctypes.cast(ms, ctypes.c_void_p), ctypes.c_void_p(stream))
+        return self.config, [float(v) for v in ms]
+
+    @property
+    def config(self):
+        """Gridding kernel configuration (cunfft_get_config)."""
+        fn = lib().cunfftf_get_config if self._sp else lib().cunfft_get_config
+        return int(fn(ctypes.byref(self._p)))
+
+    @config.setter
+    def config(self, c):
+        self._dev("cunfft_set_config", "cunfftf_set_config", ctypes.c_int(c))
 
     def ndft_rows_dev(self, fhat_ptr, rows_ptr, n_rows, out_ptr, stream=None):
         self._dev("cundft_transform_rows_dev", "cunfftf_ndft_transform_rows_dev",

@@ -188,20 +188,22 @@ __global__ __launch_bounds__(256) void k_bin_chunks(const uint32_t *__restrict__
     }
 }
 
-bool Binning::plan(const Geom &ge, int elem_bytes_, int kernel_variant)
+bool Binning::plan(const Geom &ge, int elem_bytes_, int kernel_variant, int tile_variant_)
 {
     variant = kernel_variant;
+    tile_variant = tile_variant_;
     elem_bytes = elem_bytes_;
     d = ge.d;
     usable = 0;
     const int W = ge.W;
     if (W > 18 || ge.M >= (int64_t(1) << 31)) return false;
-    static const int shapes[4][3] = {{1, 1, 1}, {256, 1, 1}, {32, 32, 1}, {8, 8, 8}};
+    int64_t shape[3];
+    tile_shape(d, planes() ? tile_variant : 0, shape);
     int64_t bytes = 2 * elem_bytes;
     ntiles = 1;
     int64_t ncells = 1;
     for (int t = 0; t < 3; ++t) {
-        T[t] = t < d ? shapes[d][t] : 1;
+        T[t] = t < d ? shape[t] : 1;
         if (t < d && ge.n[t] < T[t]) return false;
         nt[t] = t < d ? ge.n[t] / T[t] : 1;
         ntiles *= nt[t];
@@ -212,9 +214,11 @@ bool Binning::plan(const Geom &ge, int elem_bytes_, int kernel_variant)
     if (planes()) {
         // plane-tiled kernels: 32-bit grid offsets and the LDS tile must fit
         if (ncells > (int64_t(1) << 32)) return false;
-        const bool fits = elem_bytes == 4 ? tile_kernels_fit<float>(ge) : tile_kernels_fit<double>(ge);
+        const bool fits = elem_bytes == 4 ? tile_kernels_fit<float>(ge, tile_variant)
+                                          : tile_kernels_fit<double>(ge, tile_variant);
         if (!fits) return false;
-        slot_cells = elem_bytes == 4 ? tile_slot_cells<float>(ge) : tile_slot_cells<double>(ge);
+        slot_cells = elem_bytes == 4 ? tile_slot_cells<float>(ge, tile_variant)
+                                     : tile_slot_cells<double>(ge, tile_variant);
     } else {
         if (bytes > kMaxTileBytes) return false;
         if (d == 3 && W > 14) return false;

@@ -8,10 +8,29 @@ constexpr int kChunk = 256;   // nodes per chunk (one workgroup's share of a til
 
 // Grid tile per dimension: the nodes whose window starts in one tile are
 // gridded by one workgroup against the tile plus a (2m+1)-cell halo in LDS.
-template <int D> struct TileShape;
-template <> struct TileShape<1> { static constexpr int T[3] = {256, 1, 1}; };
-template <> struct TileShape<2> { static constexpr int T[3] = {32, 32, 1}; };
-template <> struct TileShape<3> { static constexpr int T[3] = {8, 8, 8}; };
+// The plane-tiled kernels (d = 2, 3) come in kTileVariants tile shapes; the
+// autotuner picks one per plan.  Variant 0 is the default.
+constexpr int kTileVariants = 3;
+template <int D, int V> struct TileShapeV;
+template <> struct TileShapeV<1, 0> { static constexpr int T[3] = {256, 1, 1}; };
+template <> struct TileShapeV<2, 0> { static constexpr int T[3] = {32, 32, 1}; };
+template <> struct TileShapeV<2, 1> { static constexpr int T[3] = {16, 16, 1}; };
+template <> struct TileShapeV<2, 2> { static constexpr int T[3] = {64, 64, 1}; };
+template <> struct TileShapeV<3, 0> { static constexpr int T[3] = {8, 8, 8}; };
+template <> struct TileShapeV<3, 1> { static constexpr int T[3] = {8, 8, 16}; };
+template <> struct TileShapeV<3, 2> { static constexpr int T[3] = {16, 16, 8}; };
+template <int D> using TileShape = TileShapeV<D, 0>;
+
+// Tile extent per axis of variant v in d dimensions (host side).
+inline void tile_shape(int d, int v, int64_t (&T)[3])
+{
+    static const int shapes[4][kTileVariants][3] = {
+        {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}},
+        {{256, 1, 1}, {256, 1, 1}, {256, 1, 1}},
+        {{32, 32, 1}, {16, 16, 1}, {64, 64, 1}},
+        {{8, 8, 8}, {8, 8, 16}, {16, 16, 8}}};
+    for (int t = 0; t < 3; ++t) T[t] = shapes[d][v][t];
+}
 
 // Work item of the tiled kernels: up to kChunk sorted nodes of one tile.
 struct Chunk {
@@ -46,6 +65,7 @@ struct Binning {
     int built = 0;
     int usable = 0;             // tiled kernels apply to this geometry
     int variant = 0;            // 0: plane-tiled kernels (d >= 2); 1: one node per lane
+    int tile_variant = 0;       // tile shape of the plane-tiled kernels (TileShapeV)
     int d = 0;
     int64_t T[3] = {1, 1, 1};   // tile extent per axis
     int64_t nt[3] = {1, 1, 1};  // tiles per axis
@@ -68,7 +88,7 @@ struct Binning {
     int elem_bytes = 8;             // bytes of one real component (4: FP32 plan, 8: FP64)
 
     // Decide tile shape for this geometry; false if the tiled path does not apply.
-    bool plan(const Geom &ge, int elem_bytes, int kernel_variant);
+    bool plan(const Geom &ge, int elem_bytes, int kernel_variant, int tile_variant = 0);
     void build(const Geom &ge, const double *x, hipStream_t s);
     void release();
     bool planes() const { return d >= 2 && variant == 0; }
@@ -93,8 +113,8 @@ bool launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, co
                            cplx_t<T> *g, hipStream_t s, hipEvent_t mid = nullptr);
 
 // Plane-tiled kernels for d = 2, 3 (grid_tile.hip).
-template <typename T> bool tile_kernels_fit(const Geom &ge);
-template <typename T> int64_t tile_slot_cells(const Geom &ge);
+template <typename T> bool tile_kernels_fit(const Geom &ge, int variant);
+template <typename T> int64_t tile_slot_cells(const Geom &ge, int variant);
 template <typename T>
 void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
                         hipStream_t s);

@@ -1,4 +1,5 @@
-// grid_tile.hip -- plane-tiled gridding kernels for d = 2, 3 on gfx950.
+// grid_tile.hpp -- plane-tiled gridding kernels for d = 2, 3 on gfx950 (templates;
+// grid_tile_f32.hip and grid_tile_f64.hip instantiate them per precision).
 //
 // Forward gridding f_j = sum_l g_l phi(x_j - l/n) (src/cunfft_kernel.cu:514-1093)
 // and adjoint gridding g_l = sum_j f_j phi(x_j - l/n) (src/cunfft_kernel.cu:
@@ -24,6 +25,7 @@
 // a destination-blocked order, and k_tile_fold sums, for every grid tile, the
 // slot blocks of the chunks whose padded tiles cover it: no atomics, no zero
 // fill, and a result that does not depend on scheduling.
+#pragma once
 #include "bin.hpp"
 
 namespace cunfft {
@@ -127,13 +129,13 @@ constexpr LaneTab<NT> make_lanes(int G, int R, int NU, int PS, int P1, int RP, i
 // Layout of the padded tile in LDS and of one node's window over the lanes.
 // PS is the plane step between the planes one wavefront visits (the number of
 // wavefronts sharing the adjoint tile, 1 for the forward).
-template <typename T, int D, int W, int PS> struct TileLayout {
+template <typename T, int D, int W, int PS, int V> struct TileLayout {
     static constexpr int PSV = PS;
     static constexpr int CB = (int)sizeof(cplx_t<T>);
     static constexpr int CPL = 16 / CB;                      // cells per lane unit
-    static constexpr int T0 = TileShape<D>::T[0];
-    static constexpr int T1 = D == 3 ? TileShape<D>::T[1] : 1;
-    static constexpr int TL = TileShape<D>::T[D - 1];
+    static constexpr int T0 = TileShapeV<D, V>::T[0];
+    static constexpr int T1 = D == 3 ? TileShapeV<D, V>::T[1] : 1;
+    static constexpr int TL = TileShapeV<D, V>::T[D - 1];
     static constexpr int P0 = T0 + W - 1;
     static constexpr int P1 = D == 3 ? T1 + W - 1 : 1;
     static constexpr int PL = TL + W - 1;
@@ -342,15 +344,15 @@ __device__ __forceinline__ double rlane_t(double v, int k)
 // weighs the sum with its row and column weights; the sixteen lanes of each
 // row are summed with DPP, and the four row sums of each node are added by
 // the node's lane after the batch.
-template <typename T, int D, int W, int NW>
+template <typename T, int D, int W, int NW, int V>
 __global__ __launch_bounds__(NW * 64) void k_tile_gather(
     const cplx_t<T> *__restrict__ g, const NodeDesc<T> *__restrict__ desc,
     const uint32_t *__restrict__ perm, const Chunk *__restrict__ chunks,
     const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f, Geom ge, BinGeom bg,
     WinConst<T> wc)
 {
-    using L = TileLayout<T, D, W, 1>;
-    using V = vec16_t<T>;
+    using L = TileLayout<T, D, W, 1, V>;
+    using VT = vec16_t<T>;
     using C = cplx_t<T>;
     constexpr int CPL = L::CPL;
     constexpr int NT = NW * 64;
@@ -387,8 +389,8 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
         const uint32_t m1 = D == 3 ? (uint32_t)ge.n[1] - 1 : 0;
         const uint32_t s1 = D == 3 ? (uint32_t)ge.n[2] : 0;
         const uint32_t s0 = D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : (uint32_t)ge.n[1];
-        V *tv = reinterpret_cast<V *>(tile);
-        const V *gv = reinterpret_cast<const V *>(g);
+        VT *tv = reinterpret_cast<VT *>(tile);
+        const VT *gv = reinterpret_cast<const VT *>(g);
         for (int idx = tid; idx < L::P0 * L::P1 * NR; idx += NT) {
             const int row = idx / NR, un = idx - row * NR;
             const int p0 = row / L::P1, p1 = row - p0 * L::P1;
@@ -431,14 +433,14 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
                 T w1v = 0, w2a = 0, w2b = 0;
 #pragma unroll
                 for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
-                    V v[PB];
+                    VT v[PB];
 #pragma unroll
                     for (int j = 0; j < PB; ++j) {
                         const int gq = g0 + j;
                         if (gq >= L::NGRP) break;
                         int a = gq * L::G + it[i].q;
                         a = a < W ? a : W - 1;   // planes past the window: weight 0 below
-                        v[j] = *reinterpret_cast<const V *>(tb + lb + (uint32_t)(a * L::PP * L::CB));
+                        v[j] = *reinterpret_cast<const VT *>(tb + lb + (uint32_t)(a * L::PP * L::CB));
                     }
                     // the node's weights are read only after its tile reads are
                     // issued, and all of them before any use: one LDS round trip
@@ -505,14 +507,14 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
 // read-add-write sequences stay in program order.  When a plane group can
 // reach past the window, the lanes concerned read and write a private dummy
 // unit past the tile.  fs holds the coefficients in sorted node order.
-template <typename T, int D, int W, int NW>
+template <typename T, int D, int W, int NW, int V>
 __global__ __launch_bounds__(NW * 64) void k_tile_spread(
     const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
     const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks,
     cplx_t<T> *__restrict__ slots, WinConst<T> wc)
 {
-    using L = TileLayout<T, D, W, NW>;
-    using V = vec16_t<T>;
+    using L = TileLayout<T, D, W, NW, V>;
+    using VT = vec16_t<T>;
     using C = cplx_t<T>;
     constexpr int CPL = L::CPL;
     constexpr int NT = NW * 64;
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
     constexpr bool DUMMY = L::NEED_DUMMY;
     constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
     constexpr int NB = batch_nodes<T, D, W>(TILE_UNITS * 16, NW, 0);
-    __shared__ __align__(16) V tile[TILE_UNITS];
+    __shared__ __align__(16) VT tile[TILE_UNITS];
     __shared__ WinBatch<T, D, W, NB> wbs[NW];
     if (blockIdx.x >= n_chunks[0]) return;
     const Chunk ch = chunks[blockIdx.x];
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
         fv = fs[ch.start + lane];
     }
     {
-        V z;
+        VT z;
         if constexpr (CPL == 2) z = make_float4(0.f, 0.f, 0.f, 0.f);
         else z = make_double2(0.0, 0.0);
         for (int idx = tid; idx < TILE_UNITS; idx += NT) tile[idx] = z;
@@ -570,7 +572,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
                 const int cw = it[i].u2 - par + 1;
 #pragma unroll
                 for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
-                    V v[PB];
+                    VT v[PB];
                     uint32_t ad[PB];
 #pragma unroll
                     for (int j = 0; j < PB; ++j) {
@@ -583,7 +585,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
                                 ad[j] = a < W ? ad[j] : dummy;
                             }
                         }
-                        v[j] = *reinterpret_cast<const V *>(tb + ad[j]);
+                        v[j] = *reinterpret_cast<const VT *>(tb + ad[j]);
                     }
                     // the node's weights are read only after its tile reads are
                     // issued, and all of them before any use: one LDS round trip
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
                             v[j].x += w0a * fux;
                             v[j].y += w0a * fuy;
                         }
-                        *reinterpret_cast<V *>(tb + ad[j]) = v[j];
+                        *reinterpret_cast<VT *>(tb + ad[j]) = v[j];
                     }
                 }
             }
@@ -674,21 +676,23 @@ __global__ __launch_bounds__(256) void k_sorted_coeffs(const cplx_t<T> *__restri
 
 // -------------------------------------------------------------- fold ----
 // g on one grid tile = sum over the chunks of the tiles whose padded region
-// covers it of the matching slot block.  One workgroup per tile; each block a
+// covers it of the matching slot block.  One workgroup per tile and slice of
+// 512 core cells (blockIdx.y); each block a
 // thread reads is contiguous over the workgroup.  The first chunk of the
 // candidate tiles of one leading-axis offset is loaded at once (a dense tile's
 // further chunks follow), and the sums run in a fixed order: the result does
 // not depend on scheduling.
-template <typename T, int D, int W, int PS>
+template <typename T, int D, int W, int PS, int V>
 __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__ slots,
                                                    const uint32_t *__restrict__ tile_count,
                                                    const uint32_t *__restrict__ chunk_start,
                                                    cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
 {
-    using L = TileLayout<T, D, W, PS>;
+    using L = TileLayout<T, D, W, PS, V>;
     using C = cplx_t<T>;
     constexpr int CORE = L::T0 * L::T1 * L::TL;
-    constexpr int CPT = (CORE + 255) / 256;
+    constexpr int CPT = CORE >= 512 ? 2 : (CORE + 255) / 256;   // cells per thread per slice
+    const int slice0 = blockIdx.y * 256 * CPT;                    // first core cell of this slice
     constexpr int QR = L::Q1 * L::QL;            // candidates per leading-axis offset
     uint32_t tk[3] = {0, 0, 0};
     {
@@ -701,7 +705,7 @@ __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__
     int r0[CPT], r1[CPT], rL[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int c = threadIdx.x + 256 * k;
+        const int c = slice0 + threadIdx.x + 256 * k;
         r0[k] = c / (L::T1 * L::TL);
         r1[k] = (c / L::TL) % L::T1;
         rL[k] = c % L::TL;
@@ -732,8 +736,8 @@ __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__
             const int boff = L::block_off(q0, q1, qL);
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
-                ok[qq][k] = cnt > 0 && threadIdx.x + 256 * k < CORE && r0[k] < E0 && r1[k] < E1 &&
-                            rL[k] < EL;
+                ok[qq][k] = cnt > 0 && slice0 + threadIdx.x + 256 * k < CORE && r0[k] < E0 &&
+                            r1[k] < E1 && rL[k] < EL;
                 li[qq][k] = boff + (r0[k] * E1 + r1[k]) * EL + rL[k];
             }
         }
@@ -778,7 +782,7 @@ __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__
     const uint32_t nL = (uint32_t)ge.n[D - 1];
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
-        if (threadIdx.x + 256 * k < CORE) {
+        if (slice0 + threadIdx.x + 256 * k < CORE) {
             uint32_t cell = (tk[0] * L::T0 + r0[k]) * (D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : nL);
             if (D == 3) cell += (tk[1] * L::T1 + r1[k]) * nL;
             cell += tk[D - 1] * L::TL + rL[k];
@@ -787,12 +791,16 @@ __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__
 }
 
 // ------------------------------------------------------------ dispatch --
-constexpr int kGatherWaves = 4;
-// Wavefronts sharing one adjoint tile, each owning every other plane: the
-// FP32 tile leaves room for three such workgroups per CU, the FP64 tile for
-// two (measured faster than one wavefront per tile, and than four).
-template <typename T> struct SpreadWaves {
-    static constexpr int value = 2;
+// Wavefronts per workgroup.  A tile small enough for two or more workgroups
+// per CU gets 4 forward wavefronts (node batches) and 2 adjoint wavefronts
+// (each owning every other plane); measured on the FP32 8^3 tile: faster than
+// 1 and 4 adjoint wavefronts.  A tile that fills the CU's LDS alone gets 8 and
+// 4, so the CU still holds enough wavefronts to cover the LDS latency.
+template <typename T, int D, int W, int V> struct Waves {
+    static constexpr int tile_bytes = TileLayout<T, D, W, 1, V>::CELLS * TileLayout<T, D, W, 1, V>::CB;
+    static constexpr bool alone = 2 * (tile_bytes + 1024) > 160 * 1024;
+    static constexpr int gather = alone ? 8 : 4;
+    static constexpr int spread = alone ? 4 : 2;
 };
 
 template <typename T> static WinConst<T> win_const(const Geom &ge)
@@ -807,126 +815,127 @@ template <typename T> static WinConst<T> win_const(const Geom &ge)
     return w;
 }
 
-template <typename T, int D, int W> static constexpr bool fits_all()
+template <typename T, int D, int W, int V> constexpr bool fits_all()
 {
-    return TileLayout<T, D, W, 1>::fits && TileLayout<T, D, W, SpreadWaves<T>::value>::fits;
+    return TileLayout<T, D, W, 1, V>::fits &&
+           TileLayout<T, D, W, Waves<T, D, W, V>::spread, V>::fits;
 }
 
-#define CUNFFT_TILE_W(CALL)                                    \
-    switch (ge.W) {                                            \
-    case 4: CALL(4); break;                                    \
-    case 6: CALL(6); break;                                    \
-    case 8: CALL(8); break;                                    \
-    case 10: CALL(10); break;                                  \
-    case 12: CALL(12); break;                                  \
-    case 14: CALL(14); break;                                  \
-    case 16: CALL(16); break;                                  \
-    case 18: CALL(18); break;                                  \
-    default: return false;                                     \
-    }
-
-template <typename T, int D, int W> static bool fits_dw() { return fits_all<T, D, W>(); }
-
-template <typename T> bool tile_kernels_fit(const Geom &ge)
+// Runs f.template run<D, W, V>() for the runtime window width, dimension and
+// tile variant; false when there is no such kernel.
+template <int D, int V, typename F> static bool with_w(int W, F &f)
 {
-    if (ge.d == 2) {
-#define FIT2(w) return fits_dw<T, 2, w>()
-        CUNFFT_TILE_W(FIT2)
-#undef FIT2
-    } else if (ge.d == 3) {
-#define FIT3(w) return fits_dw<T, 3, w>()
-        CUNFFT_TILE_W(FIT3)
-#undef FIT3
+    switch (W) {
+    case 4: return f.template run<D, 4, V>();
+    case 6: return f.template run<D, 6, V>();
+    case 8: return f.template run<D, 8, V>();
+    case 10: return f.template run<D, 10, V>();
+    case 12: return f.template run<D, 12, V>();
+    case 14: return f.template run<D, 14, V>();
+    case 16: return f.template run<D, 16, V>();
+    case 18: return f.template run<D, 18, V>();
     }
     return false;
 }
 
-template <typename T, int D, int W> static int64_t slot_cells_dw()
+template <typename F> static bool with_dwv(int d, int W, int v, F &f)
 {
-    return TileLayout<T, D, W, SpreadWaves<T>::value>::SC;
+    if (v < 0 || v >= kTileVariants) return false;
+    if (d == 2) {
+        if (v == 0) return with_w<2, 0>(W, f);
+        if (v == 1) return with_w<2, 1>(W, f);
+        return with_w<2, 2>(W, f);
+    }
+    if (d == 3) {
+        if (v == 0) return with_w<3, 0>(W, f);
+        if (v == 1) return with_w<3, 1>(W, f);
+        return with_w<3, 2>(W, f);
+    }
+    return false;
 }
 
-template <typename T> int64_t tile_slot_cells(const Geom &ge)
-{
-    auto run = [&]() -> int64_t {
-        if (ge.d == 2) {
-            switch (ge.W) {
-            case 4: return slot_cells_dw<T, 2, 4>();
-            case 6: return slot_cells_dw<T, 2, 6>();
-            case 8: return slot_cells_dw<T, 2, 8>();
-            case 10: return slot_cells_dw<T, 2, 10>();
-            case 12: return slot_cells_dw<T, 2, 12>();
-            case 14: return slot_cells_dw<T, 2, 14>();
-            case 16: return slot_cells_dw<T, 2, 16>();
-            case 18: return slot_cells_dw<T, 2, 18>();
-            }
+template <typename T> struct FitOp {
+    int64_t slot_cells = 0;
+    template <int D, int W, int V> bool run()
+    {
+        if constexpr (fits_all<T, D, W, V>()) {
+            slot_cells = TileLayout<T, D, W, Waves<T, D, W, V>::spread, V>::SC;
+            return true;
         } else {
-            switch (ge.W) {
-            case 4: return slot_cells_dw<T, 3, 4>();
-            case 6: return slot_cells_dw<T, 3, 6>();
-            case 8: return slot_cells_dw<T, 3, 8>();
-            case 10: return slot_cells_dw<T, 3, 10>();
-            case 12: return slot_cells_dw<T, 3, 12>();
-            case 14: return slot_cells_dw<T, 3, 14>();
-            case 16: return slot_cells_dw<T, 3, 16>();
-            case 18: return slot_cells_dw<T, 3, 18>();
-            }
+            return false;
         }
-        return 0;
-    };
-    return run();
+    }
+};
+
+template <typename T> bool tile_kernels_fit(const Geom &ge, int variant)
+{
+    FitOp<T> op;
+    return with_dwv(ge.d, ge.W, variant, op);
 }
 
-template <typename T, int D, int W>
-static void tile_gather_dw(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
-                           hipStream_t s)
+template <typename T> int64_t tile_slot_cells(const Geom &ge, int variant)
 {
-    if constexpr (fits_all<T, D, W>()) {
-        k_tile_gather<T, D, W, kGatherWaves><<<(unsigned)b.max_chunks, kGatherWaves * 64, 0, s>>>(
-            g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, f, ge,
-            bin_geom(b), win_const<T>(ge));
-    } else {
-        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
-    }
+    FitOp<T> op;
+    return with_dwv(ge.d, ge.W, variant, op) ? op.slot_cells : 0;
 }
 
-template <typename T, int D, int W>
-static void tile_spread_dw(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
-                           hipStream_t s, hipEvent_t mid)
-{
-    if constexpr (fits_all<T, D, W>()) {
-        cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
-        cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
-        k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
-        constexpr int NWS = SpreadWaves<T>::value;
-        k_tile_spread<T, D, W, NWS><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
-            static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
-            win_const<T>(ge));
-        if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
-        k_tile_fold<T, D, W, SpreadWaves<T>::value><<<(unsigned)b.ntiles, 256, 0, s>>>(
-            slots, b.tile_count, b.chunk_start, g, ge, bin_geom(b));
-    } else {
-        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
+template <typename T> struct GatherOp {
+    const Geom &ge;
+    const Binning &b;
+    const cplx_t<T> *g;
+    cplx_t<T> *f;
+    hipStream_t s;
+    template <int D, int W, int V> bool run()
+    {
+        if constexpr (fits_all<T, D, W, V>()) {
+            constexpr int NWG = Waves<T, D, W, V>::gather;
+            k_tile_gather<T, D, W, NWG, V><<<(unsigned)b.max_chunks, NWG * 64, 0, s>>>(
+                    g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, f,
+                    ge, bin_geom(b), win_const<T>(ge));
+            return true;
+        } else {
+            return false;
+        }
     }
-}
+};
+
+template <typename T> struct SpreadOp {
+    const Geom &ge;
+    const Binning &b;
+    const cplx_t<T> *f;
+    cplx_t<T> *g;
+    hipStream_t s;
+    hipEvent_t mid;
+    template <int D, int W, int V> bool run()
+    {
+        if constexpr (fits_all<T, D, W, V>()) {
+            constexpr int NWS = Waves<T, D, W, V>::spread;
+            cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
+            cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
+            k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
+            k_tile_spread<T, D, W, NWS, V><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
+                static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
+                win_const<T>(ge));
+            if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
+            using L = TileLayout<T, D, W, NWS, V>;
+            constexpr int CORE = L::T0 * L::T1 * L::TL;
+            constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
+            k_tile_fold<T, D, W, NWS, V><<<dim3((unsigned)b.ntiles, SLICES), 256, 0, s>>>(
+                slots, b.tile_count, b.chunk_start, g, ge, bin_geom(b));
+            return true;
+        } else {
+            return false;
+        }
+    }
+};
 
 template <typename T>
 void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
                         hipStream_t s)
 {
-    auto run = [&]() -> bool {
-        if (ge.d == 2) {
-#define G2(w) tile_gather_dw<T, 2, w>(ge, b, g, f, s)
-            CUNFFT_TILE_W(G2)
-#undef G2
-        } else {
-#define G3(w) tile_gather_dw<T, 3, w>(ge, b, g, f, s)
-            CUNFFT_TILE_W(G3)
-#undef G3
-        }
-        return true;
-    };
-    if (!run()) throw Error(CUNFFT_ERR_UNSUPPORTED, "tiled gridding needs m <= 8");
+    GatherOp<T> op{ge, b, g, f, s};
+    if (!with_dwv(ge.d, ge.W, b.tile_variant, op))
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "no tiled gather for this geometry");
     CUNFFT_HIP(hipGetLastError());
 }
 
@@ -934,33 +943,10 @@ template <typename T>
 void launch_tile_spread(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
                         hipStream_t s, hipEvent_t mid)
 {
-    auto run = [&]() -> bool {
-        if (ge.d == 2) {
-#define S2(w) tile_spread_dw<T, 2, w>(ge, b, f, g, s, mid)
-            CUNFFT_TILE_W(S2)
-#undef S2
-        } else {
-#define S3(w) tile_spread_dw<T, 3, w>(ge, b, f, g, s, mid)
-            CUNFFT_TILE_W(S3)
-#undef S3
-        }
-        return true;
-    };
-    if (!run()) throw Error(CUNFFT_ERR_UNSUPPORTED, "tiled gridding needs m <= 8");
+    SpreadOp<T> op{ge, b, f, g, s, mid};
+    if (!with_dwv(ge.d, ge.W, b.tile_variant, op))
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "no tiled spread for this geometry");
     CUNFFT_HIP(hipGetLastError());
 }
-
-template bool tile_kernels_fit<float>(const Geom &);
-template bool tile_kernels_fit<double>(const Geom &);
-template int64_t tile_slot_cells<float>(const Geom &);
-template int64_t tile_slot_cells<double>(const Geom &);
-template void launch_tile_gather<float>(const Geom &, const Binning &, const float2 *, float2 *,
-                                        hipStream_t);
-template void launch_tile_gather<double>(const Geom &, const Binning &, const double2 *,
-                                         double2 *, hipStream_t);
-template void launch_tile_spread<float>(const Geom &, const Binning &, const float2 *, float2 *,
-                                        hipStream_t, hipEvent_t);
-template void launch_tile_spread<double>(const Geom &, const Binning &, const double2 *,
-                                         double2 *, hipStream_t, hipEvent_t);
 
 }  // namespace cunfft

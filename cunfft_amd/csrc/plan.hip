@@ -23,6 +23,8 @@
 
 namespace cunfft {
 
+static_assert(kTileVariants == CUNFFT_TILE_SHAPES, "tile shapes of the C ABI and the kernels");
+
 void hip_check(hipError_t e, const char *what)
 {
     if (e != hipSuccess) {
@@ -114,6 +116,7 @@ template <typename T> struct Impl {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     Binning bins;           // node binning for the tiled gridding kernels
     bool nodes_dirty = true;
+    int config = 0;         // gridding kernel configuration (cunfft_set_config)
 };
 
 template <typename T> static Impl<T> *impl_of(typename Traits<T>::plan *p)
@@ -256,7 +259,9 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
         ge.invb[t] = t < d ? 1.0 / p->b[t] : 0.0;
         ge.nrm[t] = t < d ? 1.0 / sqrt(M_PI * p->b[t]) : 0.0;
     }
-    im->bins.plan(ge, (int)sizeof(T), (p->flags & CUNFFT_NODE_LANES) ? 1 : 0);
+    im->config = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
+    im->bins.plan(ge, (int)sizeof(T), im->config == kTileVariants ? 1 : 0,
+                  im->config == kTileVariants ? 0 : im->config);
     // roll-off tables by pruned index p: exp(fac_t k^2), k = p or p - N_t
     std::vector<double> roc;
     for (int t = 0; t < d; ++t) {
@@ -448,6 +453,90 @@ static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
         *t_fft = b * 1e-3;
         *t_fold = c * 1e-3;
     }
+}
+
+// ------------------------------------------------------------ autotuner --
+// Gridding configuration c: c < kTileVariants is the plane-tiled kernel pair
+// with tile shape c, c == kTileVariants the node-lane kernels.  Re-plans the
+// binning (its buffers are re-allocated by the next transform); false when the
+// tiled kernels do not apply to this configuration.
+template <typename T> static bool plan_config(Impl<T> *im, int c)
+{
+    im->bins.release();
+    im->nodes_dirty = true;
+    const bool lanes = c == kTileVariants;
+    return im->bins.plan(im->geom, (int)sizeof(T), lanes ? 1 : 0, lanes ? 0 : c);
+}
+
+template <typename T> static bool config_applies(int d, int c)
+{
+    return c >= 0 && c <= kTileVariants && (d >= 2 || c == 0);
+}
+
+// Times every configuration that applies on the plan's current nodes (best of
+// reps forward and/or adjoint passes over zero data, event-timed on s) and
+// keeps the fastest.  The binning of each candidate is built inside the loop
+// and not timed: it is paid once per node set.
+template <typename T>
+static void autotune(typename Traits<T>::plan *p, Impl<T> *im, int dirs, int reps, float *ms_out,
+                     hipStream_t s)
+{
+    if (dirs & ~3 || !dirs) throw Error(CUNFFT_ERR_INVALID_ARG, "directions must be 1, 2 or 3");
+    if (reps <= 0) reps = 5;
+    void *fh = nullptr, *f = nullptr;
+    float ms[kTileVariants + 1];
+    int best = im->config;
+    float best_ms = 0;
+    try {
+        CUNFFT_HIP(hipMalloc(&fh, sizeof(cplx_t<T>) * p->N_total));
+        CUNFFT_HIP(hipMalloc(&f, sizeof(cplx_t<T>) * p->M_total));
+        CUNFFT_HIP(hipMemsetAsync(fh, 0, sizeof(cplx_t<T>) * p->N_total, s));
+        CUNFFT_HIP(hipMemsetAsync(f, 0, sizeof(cplx_t<T>) * p->M_total, s));
+        auto pass = [&] {
+            double a, b, c;
+            if (dirs & 1)
+                run_forward<T>(p, im, static_cast<const cplx_t<T> *>(fh),
+                               static_cast<cplx_t<T> *>(f), s, false, &a, &b);
+            if (dirs & 2)
+                run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(f),
+                               static_cast<cplx_t<T> *>(fh), s, false, &a, &b, &c);
+        };
+        for (int c = 0; c <= kTileVariants; ++c) {
+            ms[c] = -1.0f;
+            if (!config_applies<T>(p->d, c) || !plan_config<T>(im, c)) continue;
+            ensure_bins<T>(p, im, p->x_gpu, s);
+            pass();
+            float t_min = 0;
+            for (int r = 0; r < reps; ++r) {
+                CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+                pass();
+                CUNFFT_HIP(hipEventRecord(im->ev[1], s));
+                CUNFFT_HIP(hipEventSynchronize(im->ev[1]));
+                float t = 0;
+                CUNFFT_HIP(hipEventElapsedTime(&t, im->ev[0], im->ev[1]));
+                t_min = r == 0 ? t : std::min(t_min, t);
+            }
+            ms[c] = t_min;
+            if (best_ms == 0 || t_min < best_ms) {
+                best_ms = t_min;
+                best = c;
+            }
+        }
+        CUNFFT_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        if (fh) (void)hipFree(fh);
+        if (f) (void)hipFree(f);
+        im->config = best;
+        plan_config<T>(im, best);
+        throw;
+    }
+    (void)hipFree(fh);
+    (void)hipFree(f);
+    im->config = best;
+    plan_config<T>(im, best);
+    if (ms_out)
+        for (int c = 0; c <= kTileVariants; ++c) ms_out[c] = ms[c];
 }
 
 template <typename T>
@@ -774,6 +863,47 @@ int api_ndft_dev(typename Traits<T>::plan *p, bool adjoint, const void *in, cons
     });
 }
 
+template <typename T>
+int api_autotune(typename Traits<T>::plan *p, int dirs, int reps, float *ms_out, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        autotune<T>(p, im, dirs, reps, ms_out, pick_stream(stream, im->stream));
+    });
+}
+
+template <typename T> int api_get_config(const typename Traits<T>::plan *p)
+{
+    if (!p || !p->impl) return -1;
+    return static_cast<const Impl<T> *>(p->impl)->config;
+}
+
+// A configuration that does not exist or does not fit is refused with its
+// status and leaves the plan as it was (the status is not recorded on it).
+template <typename T> int api_set_config(typename Traits<T>::plan *p, int c)
+{
+    if (!p) return CUNFFT_ERR_INVALID_ARG;
+    if (p->status != CUNFFT_SUCCESS) return p->status;
+    if (!p->impl) return CUNFFT_ERR_STATE;
+    Impl<T> *im = static_cast<Impl<T> *>(p->impl);
+    if (!config_applies<T>(p->d, c)) {
+        report("no such gridding configuration for this plan");
+        return CUNFFT_ERR_INVALID_ARG;
+    }
+    Binning probe;
+    const bool lanes = c == kTileVariants;
+    if (!probe.plan(im->geom, (int)sizeof(T), lanes ? 1 : 0, lanes ? 0 : c)) {
+        report("the tiled kernels of this configuration do not fit the plan's window and grid");
+        return CUNFFT_ERR_UNSUPPORTED;
+    }
+    return guard(p, [&] {
+        CUNFFT_HIP(hipDeviceSynchronize());
+        plan_config<T>(im, c);
+        im->config = c;
+    });
+}
+
 template <typename T> void api_init_plan(int dim, uint_t *n, cunfftHandle *h)
 {
     if (!h) return;
@@ -942,6 +1072,12 @@ int cunfft_backward_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat,
 {
     return api_backward_dev<double>(p, fhat, grad_f, grad_fhat, grad_x, s);
 }
+int cunfft_autotune(cunfft_plan *p, int directions, int reps, float *ms, void *s)
+{
+    return api_autotune<double>(p, directions, reps, ms, s);
+}
+int cunfft_get_config(const cunfft_plan *p) { return api_get_config<double>(p); }
+int cunfft_set_config(cunfft_plan *p, int c) { return api_set_config<double>(p, c); }
 int cundft_transform_rows_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const long long *rows,
                               long long n_rows, cunfftDoubleComplex *out, void *s)
 {
@@ -1047,6 +1183,12 @@ int cunfftf_backward_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat,
 {
     return api_backward_dev<float>(p, fhat, grad_f, grad_fhat, grad_x, s);
 }
+int cunfftf_autotune(cunfftf_plan *p, int directions, int reps, float *ms, void *s)
+{
+    return api_autotune<float>(p, directions, reps, ms, s);
+}
+int cunfftf_get_config(const cunfftf_plan *p) { return api_get_config<float>(p); }
+int cunfftf_set_config(cunfftf_plan *p, int c) { return api_set_config<float>(p, c); }
 int cunfftf_ndft_transform_rows_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat,
                                     const long long *rows, long long n_rows, cunfftFloatComplex *out,
                                     void *s)

@@ -244,6 +244,31 @@ int cunfft_grad_x_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
 int cunfft_backward_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
                         const cunfftDoubleComplex *grad_f_dev, cunfftDoubleComplex *grad_fhat_dev,
                         double *grad_x_dev, void *stream);
+/* Gridding kernel configurations (no counterpart in the reference, whose
+ * tile shape and kernels are fixed at compile time, src/cunfft_kernel.cu).
+ * Configuration c < CUNFFT_TILE_SHAPES is the plane-tiled kernel pair (one
+ * node per wavefront instruction, lanes over the rows of the node's window
+ * planes) on tile shape c -- 2D: 32x32, 16x16, 64x64; 3D: 8x8x8, 8x8x16,
+ * 16x16x8; configuration CUNFFT_TILE_SHAPES is the node-lane pair (64 nodes
+ * per wavefront instruction; 32x32 / 8x8x8 tiles).  1D plans have the one
+ * configuration 0.  The default is 0 (CUNFFT_TILE_SHAPES with the
+ * CUNFFT_NODE_LANES flag). */
+#define CUNFFT_TILE_SHAPES 3
+#define CUNFFT_TUNE_CONFIGS (CUNFFT_TILE_SHAPES + 1)
+#define CUNFFT_TUNE_FORWARD 1
+#define CUNFFT_TUNE_ADJOINT 2
+/* Autotuner: times every configuration that applies on the plan's current
+ * nodes (set them first), over `directions` (CUNFFT_TUNE_FORWARD and/or
+ * CUNFFT_TUNE_ADJOINT), best of `reps` (<= 0: 5), and keeps the fastest.
+ * ms_out (may be NULL) receives CUNFFT_TUNE_CONFIGS times in milliseconds,
+ * -1 where a configuration does not apply.  Allocates and frees temporaries,
+ * runs on `stream` and synchronizes it. */
+int cunfft_autotune(cunfft_plan *plan, int directions, int reps, float *ms_out, void *stream);
+/* The plan's gridding configuration (-1 for an uninitialised plan). */
+int cunfft_get_config(const cunfft_plan *plan);
+/* Select a configuration; the binning is rebuilt by the next transform.
+ * Synchronizes the device. */
+int cunfft_set_config(cunfft_plan *plan, int config);
 /* Direct NDFT on the GPU in FP64 for the listed outputs only (rows of f or
  * entries of fhat by flat index); rows == NULL means all outputs. */
 int cundft_transform_rows_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
@@ -291,6 +316,9 @@ int cunfftf_grad_x_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
 int cunfftf_backward_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                          const cunfftFloatComplex *grad_f_dev, cunfftFloatComplex *grad_fhat_dev,
                          float *grad_x_dev, void *stream);
+int cunfftf_autotune(cunfftf_plan *plan, int directions, int reps, float *ms_out, void *stream);
+int cunfftf_get_config(const cunfftf_plan *plan);
+int cunfftf_set_config(cunfftf_plan *plan, int config);
 int cunfftf_ndft_transform_rows_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                                     const long long *rows_dev, long long n_rows,
                                     cunfftFloatComplex *out_dev, void *stream);
@@ -333,6 +361,9 @@ typedef float dTyp;
 #define cunfft_adjoint_dev cunfftf_adjoint_dev
 #define cunfft_grad_x_dev cunfftf_grad_x_dev
 #define cunfft_backward_dev cunfftf_backward_dev
+#define cunfft_autotune cunfftf_autotune
+#define cunfft_get_config cunfftf_get_config
+#define cunfft_set_config cunfftf_set_config
 #define cundft_transform_rows_dev cunfftf_ndft_transform_rows_dev
 #define cundft_adjoint_cols_dev cunfftf_ndft_adjoint_cols_dev
 #else

@@ -153,10 +153,10 @@ def lib():
         getattr(L, s).argtypes = [PP, ctypes.c_int, ctypes.c_int, vp, vp]
         getattr(L, s).restype = ctypes.c_int
     for s in ("cunfft_get_config", "cunfftf_get_config"):
-        getattr(L, s).argtypes = [PP]
+        getattr(L, s).argtypes = [PP, ctypes.c_int]
         getattr(L, s).restype = ctypes.c_int
     for s in ("cunfft_set_config", "cunfftf_set_config"):
-        getattr(L, s).argtypes = [PP, ctypes.c_int]
+        getattr(L, s).argtypes = [PP, ctypes.c_int, ctypes.c_int]
         getattr(L, s).restype = ctypes.c_int
     L.cunfft_last_message.restype = ctypes.c_char_p
     L.cunfft_error_string.restype = ctypes.c_char_p
@@ -398,22 +398,30 @@ class Plan:
     # ------------------------------------------------------------ autotuner
     def autotune(self, directions=TUNE_FORWARD | TUNE_ADJOINT, reps=5, stream=None):
         """Time the gridding configurations on the plan's current nodes and keep
-        the fastest (cunfft_autotune).  Returns (chosen config, times in ms per
-        config, -1 where a configuration does not apply)."""
-        ms = (ctypes.c_float * TUNE_CONFIGS)()
+        the fastest per direction (cunfft_autotune).  Returns ((forward config,
+        adjoint config), forward times, adjoint times) with times in ms per
+        configuration, -1 where a configuration does not apply or a direction
+        was not tuned."""
+        ms = (ctypes.c_float * (2 * TUNE_CONFIGS))()
         self._dev("cunfft_autotune", "cunfftf_autotune", ctypes.c_int(directions),
                   ctypes.c_int(reps), ctypes.cast(ms, ctypes.c_void_p), ctypes.c_void_p(stream))
-        return self.config, [float(v) for v in ms]
+        v = [float(t) for t in ms]
+        return self.config, v[:TUNE_CONFIGS], v[TUNE_CONFIGS:]
 
     @property
     def config(self):
-        """Gridding kernel configuration (cunfft_get_config)."""
+        """(forward, adjoint) gridding kernel configurations (cunfft_get_config)."""
         fn = lib().cunfftf_get_config if self._sp else lib().cunfft_get_config
-        return int(fn(ctypes.byref(self._p)))
+        return (int(fn(ctypes.byref(self._p), TUNE_FORWARD)),
+                int(fn(ctypes.byref(self._p), TUNE_ADJOINT)))
 
     @config.setter
     def config(self, c):
-        self._dev("cunfft_set_config", "cunfftf_set_config", ctypes.c_int(c))
+        """An int sets both directions; (forward, adjoint) sets each."""
+        pairs = [(TUNE_FORWARD | TUNE_ADJOINT, c)] if isinstance(c, int) else \
+            [(TUNE_FORWARD, c[0]), (TUNE_ADJOINT, c[1])]
+        for dirs, v in pairs:
+            self._dev("cunfft_set_config", "cunfftf_set_config", ctypes.c_int(dirs), ctypes.c_int(v))
 
     def ndft_rows_dev(self, fhat_ptr, rows_ptr, n_rows, out_ptr, stream=None):
         self._dev("cundft_transform_rows_dev", "cunfftf_ndft_transform_rows_dev",

@@ -114,9 +114,13 @@ template <typename T> struct Impl {
     double *roc = nullptr;  // device roll-off tables, N_t entries per axis
     int64_t rocoff[3] = {0, 0, 0};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    Binning bins;           // node binning for the tiled gridding kernels
-    bool nodes_dirty = true;
-    int config = 0;         // gridding kernel configuration (cunfft_set_config)
+    // Node binning of the forward [0] and adjoint [1] gridding kernels; the
+    // adjoint shares [0] while both directions use one configuration.
+    Binning bins[2];
+    bool dirty[2] = {true, true};
+    int config[2] = {0, 0};  // gridding kernel configuration per direction
+    int slot(int dir) const { return config[1] == config[0] ? 0 : dir; }
+    void nodes_changed() { dirty[0] = dirty[1] = true; }
 };
 
 template <typename T> static Impl<T> *impl_of(typename Traits<T>::plan *p)
@@ -191,13 +195,38 @@ template <typename T> static void init_device_para(typename Traits<T>::plan *p)
     CUNFFT_HIP(hipMemset(p->fhat_gpu, 0, sizeof(C2) * p->N_total));
     CUNFFT_HIP(hipMemset(p->x_gpu, 0, sizeof(double) * p->M_total * p->d));
     CUNFFT_HIP(hipMemset(p->f_gpu, 0, sizeof(C2) * p->M_total));
-    if (p->impl) static_cast<Impl<T> *>(p->impl)->nodes_dirty = true;
+    if (p->impl) static_cast<Impl<T> *>(p->impl)->nodes_changed();
 }
 
 // src/cunfft.cu:60-76: exp(-l^2 / b) for l = 0 .. 2m+1 (evaluated directly).
 static void fg_exp_l(double *out, double b, int offset, int m)
 {
     for (int l = 0; l < 2 * m + 2; ++l) out[offset + l] = exp(-(double)l * (double)l / b);
+}
+
+// Gridding configuration c: c < kTileVariants is the plane-tiled kernel pair
+// with tile shape c, c == kTileVariants the node-lane kernels.
+static bool config_exists(int d, int c) { return c >= 0 && c <= kTileVariants && (d >= 2 || c == 0); }
+
+static bool plan_bins(Binning &b, const Geom &ge, int elem_bytes, int c)
+{
+    const bool lanes = c == kTileVariants;
+    return b.plan(ge, elem_bytes, lanes ? 1 : 0, lanes ? 0 : c);
+}
+
+// Forward configuration cf, adjoint ca: re-plans the binnings (their buffers
+// are re-allocated by the next transform); false when the tiled kernels of
+// either do not apply (that direction then uses the untiled kernels).
+template <typename T> static bool plan_configs(Impl<T> *im, int cf, int ca)
+{
+    im->bins[0].release();
+    im->bins[1].release();
+    im->nodes_changed();
+    im->config[0] = cf;
+    im->config[1] = ca;
+    bool ok = plan_bins(im->bins[0], im->geom, (int)sizeof(T), cf);
+    if (ca != cf) ok = plan_bins(im->bins[1], im->geom, (int)sizeof(T), ca) && ok;
+    return ok;
 }
 
 // src/cunfft.cu:334-378 (cunfft_initHelp) on the fields set by init/initGuru.
@@ -259,9 +288,8 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
         ge.invb[t] = t < d ? 1.0 / p->b[t] : 0.0;
         ge.nrm[t] = t < d ? 1.0 / sqrt(M_PI * p->b[t]) : 0.0;
     }
-    im->config = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
-    im->bins.plan(ge, (int)sizeof(T), im->config == kTileVariants ? 1 : 0,
-                  im->config == kTileVariants ? 0 : im->config);
+    const int c0 = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
+    plan_configs<T>(im, c0, c0);
     // roll-off tables by pruned index p: exp(fac_t k^2), k = p or p - N_t
     std::vector<double> roc;
     for (int t = 0; t < d; ++t) {
@@ -307,7 +335,8 @@ template <typename T> static void finalize(typename Traits<T>::plan *p)
     p->n = p->N = nullptr;
     free_device<T>(p);
     if (im) {
-        im->bins.release();
+        im->bins[0].release();
+        im->bins[1].release();
         if (im->roc) (void)hipFree(im->roc);
         for (int k = 0; k < 4; ++k)
             if (im->ev[k]) (void)hipEventDestroy(im->ev[k]);
@@ -384,12 +413,17 @@ template <typename T> static FftTables *fft_tables(typename Traits<T>::plan *p, 
     return static_cast<FftTables *>(h);
 }
 
+// Binning of direction dir (0 forward, 1 adjoint), built on s if the nodes
+// changed since.
 template <typename T>
-static void ensure_bins(typename Traits<T>::plan *p, Impl<T> *im, const double *x, hipStream_t s)
+static const Binning &ensure_bins(Impl<T> *im, int dir, const double *x, hipStream_t s)
 {
-    if (!im->nodes_dirty) return;
-    im->bins.build(im->geom, x, s);
-    im->nodes_dirty = false;
+    const int k = im->slot(dir);
+    if (im->dirty[k]) {
+        im->bins[k].build(im->geom, x, s);
+        im->dirty[k] = false;
+    }
+    return im->bins[k];
 }
 
 // f = A fhat.  Returns the event-timed (fft, conv) seconds when timing.
@@ -405,8 +439,7 @@ static void run_forward(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
     if (p->flags & CUNFFT_PLAIN_GATHER) {
         launch_gather_plain<T>(im->geom, g, x, f, s);
     } else {
-        ensure_bins<T>(p, im, x, s);
-        launch_gather_binned<T>(im->geom, im->bins, g, x, f, s);
+        launch_gather_binned<T>(im->geom, ensure_bins<T>(im, 0, x, s), g, x, f, s);
     }
     if (timing) {
         CUNFFT_HIP(hipEventRecord(im->ev[2], s));
@@ -434,10 +467,9 @@ static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
         launch_zero<T>(g, p->n_total, s);
         launch_scatter_atomic<T>(im->geom, x, f, g, s);
     } else {
-        ensure_bins<T>(p, im, x, s);
+        const Binning &b = ensure_bins<T>(im, 1, x, s);
         if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
-        folded = launch_scatter_binned<T>(im->geom, im->bins, x, f, g, s,
-                                          timing ? im->ev[3] : nullptr);
+        folded = launch_scatter_binned<T>(im->geom, b, x, f, g, s, timing ? im->ev[3] : nullptr);
     }
     if (timing && !folded) CUNFFT_HIP(hipEventRecord(im->ev[3], s));
     if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], s));
@@ -456,87 +488,75 @@ static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
 }
 
 // ------------------------------------------------------------ autotuner --
-// Gridding configuration c: c < kTileVariants is the plane-tiled kernel pair
-// with tile shape c, c == kTileVariants the node-lane kernels.  Re-plans the
-// binning (its buffers are re-allocated by the next transform); false when the
-// tiled kernels do not apply to this configuration.
-template <typename T> static bool plan_config(Impl<T> *im, int c)
-{
-    im->bins.release();
-    im->nodes_dirty = true;
-    const bool lanes = c == kTileVariants;
-    return im->bins.plan(im->geom, (int)sizeof(T), lanes ? 1 : 0, lanes ? 0 : c);
-}
-
-template <typename T> static bool config_applies(int d, int c)
-{
-    return c >= 0 && c <= kTileVariants && (d >= 2 || c == 0);
-}
 
 // Times every configuration that applies on the plan's current nodes (best of
-// reps forward and/or adjoint passes over zero data, event-timed on s) and
-// keeps the fastest.  The binning of each candidate is built inside the loop
-// and not timed: it is paid once per node set.
+// reps forward and adjoint passes over zero data, event-timed on s) and keeps
+// the fastest per tuned direction.  The binning of each candidate is built
+// before its timed passes: it is paid once per node set.  ms_out: forward
+// times of the configurations, then adjoint times (-1: not timed).
 template <typename T>
 static void autotune(typename Traits<T>::plan *p, Impl<T> *im, int dirs, int reps, float *ms_out,
                      hipStream_t s)
 {
+    constexpr int NC = kTileVariants + 1;
     if (dirs & ~3 || !dirs) throw Error(CUNFFT_ERR_INVALID_ARG, "directions must be 1, 2 or 3");
     if (reps <= 0) reps = 5;
     void *fh = nullptr, *f = nullptr;
-    float ms[kTileVariants + 1];
-    int best = im->config;
-    float best_ms = 0;
+    float ms[2][NC];
+    int best[2] = {im->config[0], im->config[1]};
+    auto finish = [&] {
+        (void)hipStreamSynchronize(s);
+        if (fh) (void)hipFree(fh);
+        if (f) (void)hipFree(f);
+        plan_configs<T>(im, best[0], best[1]);
+    };
     try {
         CUNFFT_HIP(hipMalloc(&fh, sizeof(cplx_t<T>) * p->N_total));
         CUNFFT_HIP(hipMalloc(&f, sizeof(cplx_t<T>) * p->M_total));
         CUNFFT_HIP(hipMemsetAsync(fh, 0, sizeof(cplx_t<T>) * p->N_total, s));
         CUNFFT_HIP(hipMemsetAsync(f, 0, sizeof(cplx_t<T>) * p->M_total, s));
-        auto pass = [&] {
-            double a, b, c;
-            if (dirs & 1)
-                run_forward<T>(p, im, static_cast<const cplx_t<T> *>(fh),
-                               static_cast<cplx_t<T> *>(f), s, false, &a, &b);
-            if (dirs & 2)
-                run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(f),
-                               static_cast<cplx_t<T> *>(fh), s, false, &a, &b, &c);
-        };
-        for (int c = 0; c <= kTileVariants; ++c) {
-            ms[c] = -1.0f;
-            if (!config_applies<T>(p->d, c) || !plan_config<T>(im, c)) continue;
-            ensure_bins<T>(p, im, p->x_gpu, s);
-            pass();
-            float t_min = 0;
-            for (int r = 0; r < reps; ++r) {
-                CUNFFT_HIP(hipEventRecord(im->ev[0], s));
-                pass();
-                CUNFFT_HIP(hipEventRecord(im->ev[1], s));
-                CUNFFT_HIP(hipEventSynchronize(im->ev[1]));
-                float t = 0;
-                CUNFFT_HIP(hipEventElapsedTime(&t, im->ev[0], im->ev[1]));
-                t_min = r == 0 ? t : std::min(t_min, t);
-            }
-            ms[c] = t_min;
-            if (best_ms == 0 || t_min < best_ms) {
-                best_ms = t_min;
-                best = c;
+        float best_ms[2] = {-1.0f, -1.0f};
+        for (int c = 0; c < NC; ++c) {
+            ms[0][c] = ms[1][c] = -1.0f;
+            if (!config_exists(p->d, c) || !plan_configs<T>(im, c, c)) continue;
+            for (int dir = 0; dir < 2; ++dir) {
+                if (!(dirs & (1 << dir))) continue;
+                double a, b, e;
+                auto pass = [&] {
+                    if (dir == 0)
+                        run_forward<T>(p, im, static_cast<const cplx_t<T> *>(fh),
+                                       static_cast<cplx_t<T> *>(f), s, false, &a, &b);
+                    else
+                        run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(f),
+                                       static_cast<cplx_t<T> *>(fh), s, false, &a, &b, &e);
+                };
+                pass();   // builds the binning, warms up
+                float t_min = 0;
+                for (int r = 0; r < reps; ++r) {
+                    CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+                    pass();
+                    CUNFFT_HIP(hipEventRecord(im->ev[1], s));
+                    CUNFFT_HIP(hipEventSynchronize(im->ev[1]));
+                    float t = 0;
+                    CUNFFT_HIP(hipEventElapsedTime(&t, im->ev[0], im->ev[1]));
+                    t_min = r == 0 ? t : std::min(t_min, t);
+                }
+                ms[dir][c] = t_min;
+                if (best_ms[dir] < 0 || t_min < best_ms[dir]) {
+                    best_ms[dir] = t_min;
+                    best[dir] = c;
+                }
             }
         }
         CUNFFT_HIP(hipStreamSynchronize(s));
     } catch (...) {
-        (void)hipStreamSynchronize(s);
-        if (fh) (void)hipFree(fh);
-        if (f) (void)hipFree(f);
-        im->config = best;
-        plan_config<T>(im, best);
+        finish();
         throw;
     }
-    (void)hipFree(fh);
-    (void)hipFree(f);
-    im->config = best;
-    plan_config<T>(im, best);
+    finish();
     if (ms_out)
-        for (int c = 0; c <= kTileVariants; ++c) ms_out[c] = ms[c];
+        for (int dir = 0; dir < 2; ++dir)
+            for (int c = 0; c < NC; ++c) ms_out[dir * NC + c] = ms[dir][c];
 }
 
 template <typename T>
@@ -640,7 +660,7 @@ template <typename T> void api_copy_in(typename Traits<T>::plan *p, bool adjoint
                             hipMemcpyHostToDevice);
         t += timed_copy(im, timing, p->x_gpu, p->x, sizeof(double) * p->M_total * p->d,
                         hipMemcpyHostToDevice);
-        im->nodes_dirty = true;
+        im->nodes_changed();
         p->CUNFFTTimes.time_COPY_IN = t;
     });
 }
@@ -767,7 +787,7 @@ int api_set_nodes_dev(typename Traits<T>::plan *p, const double *x_dev, void *st
         if (!x_dev) throw Error(CUNFFT_ERR_INVALID_ARG, "x is NULL");
         CUNFFT_HIP(hipMemcpyAsync(p->x_gpu, x_dev, sizeof(double) * p->M_total * p->d,
                                   hipMemcpyDeviceToDevice, pick_stream(stream, im->stream)));
-        im->nodes_dirty = true;
+        im->nodes_changed();
     });
 }
 
@@ -866,6 +886,10 @@ int api_ndft_dev(typename Traits<T>::plan *p, bool adjoint, const void *in, cons
 template <typename T>
 int api_autotune(typename Traits<T>::plan *p, int dirs, int reps, float *ms_out, void *stream)
 {
+    if (p && p->status == CUNFFT_SUCCESS && (dirs & ~3 || !dirs)) {
+        report("directions must be 1, 2 or 3");
+        return CUNFFT_ERR_INVALID_ARG;   // refused; the plan stays usable
+    }
     return guard(p, [&] {
         Impl<T> *im = impl_of<T>(p);
         need_device<T>(p);
@@ -873,34 +897,36 @@ int api_autotune(typename Traits<T>::plan *p, int dirs, int reps, float *ms_out,
     });
 }
 
-template <typename T> int api_get_config(const typename Traits<T>::plan *p)
+template <typename T> int api_get_config(const typename Traits<T>::plan *p, int direction)
 {
-    if (!p || !p->impl) return -1;
-    return static_cast<const Impl<T> *>(p->impl)->config;
+    if (!p || !p->impl || (direction != 1 && direction != 2)) return -1;
+    return static_cast<const Impl<T> *>(p->impl)->config[direction - 1];
 }
 
 // A configuration that does not exist or does not fit is refused with its
 // status and leaves the plan as it was (the status is not recorded on it).
-template <typename T> int api_set_config(typename Traits<T>::plan *p, int c)
+template <typename T> int api_set_config(typename Traits<T>::plan *p, int dirs, int c)
 {
     if (!p) return CUNFFT_ERR_INVALID_ARG;
     if (p->status != CUNFFT_SUCCESS) return p->status;
     if (!p->impl) return CUNFFT_ERR_STATE;
     Impl<T> *im = static_cast<Impl<T> *>(p->impl);
-    if (!config_applies<T>(p->d, c)) {
+    if (dirs & ~3 || !dirs) {
+        report("directions must be 1, 2 or 3");
+        return CUNFFT_ERR_INVALID_ARG;
+    }
+    if (!config_exists(p->d, c)) {
         report("no such gridding configuration for this plan");
         return CUNFFT_ERR_INVALID_ARG;
     }
     Binning probe;
-    const bool lanes = c == kTileVariants;
-    if (!probe.plan(im->geom, (int)sizeof(T), lanes ? 1 : 0, lanes ? 0 : c)) {
+    if (!plan_bins(probe, im->geom, (int)sizeof(T), c)) {
         report("the tiled kernels of this configuration do not fit the plan's window and grid");
         return CUNFFT_ERR_UNSUPPORTED;
     }
     return guard(p, [&] {
         CUNFFT_HIP(hipDeviceSynchronize());
-        plan_config<T>(im, c);
-        im->config = c;
+        plan_configs<T>(im, (dirs & 1) ? c : im->config[0], (dirs & 2) ? c : im->config[1]);
     });
 }
 
@@ -1076,8 +1102,14 @@ int cunfft_autotune(cunfft_plan *p, int directions, int reps, float *ms, void *s
 {
     return api_autotune<double>(p, directions, reps, ms, s);
 }
-int cunfft_get_config(const cunfft_plan *p) { return api_get_config<double>(p); }
-int cunfft_set_config(cunfft_plan *p, int c) { return api_set_config<double>(p, c); }
+int cunfft_get_config(const cunfft_plan *p, int direction)
+{
+    return api_get_config<double>(p, direction);
+}
+int cunfft_set_config(cunfft_plan *p, int directions, int c)
+{
+    return api_set_config<double>(p, directions, c);
+}
 int cundft_transform_rows_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const long long *rows,
                               long long n_rows, cunfftDoubleComplex *out, void *s)
 {
@@ -1187,8 +1219,14 @@ int cunfftf_autotune(cunfftf_plan *p, int directions, int reps, float *ms, void 
 {
     return api_autotune<float>(p, directions, reps, ms, s);
 }
-int cunfftf_get_config(const cunfftf_plan *p) { return api_get_config<float>(p); }
-int cunfftf_set_config(cunfftf_plan *p, int c) { return api_set_config<float>(p, c); }
+int cunfftf_get_config(const cunfftf_plan *p, int direction)
+{
+    return api_get_config<float>(p, direction);
+}
+int cunfftf_set_config(cunfftf_plan *p, int directions, int c)
+{
+    return api_set_config<float>(p, directions, c);
+}
 int cunfftf_ndft_transform_rows_dev(cunfftf_plan *p, const cunfftFloatComplex *fhat,
                                     const long long *rows, long long n_rows, cunfftFloatComplex *out,
                                     void *s)

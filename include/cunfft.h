@@ -258,17 +258,24 @@ int cunfft_backward_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
 #define CUNFFT_TUNE_FORWARD 1
 #define CUNFFT_TUNE_ADJOINT 2
 /* Autotuner: times every configuration that applies on the plan's current
- * nodes (set them first), over `directions` (CUNFFT_TUNE_FORWARD and/or
- * CUNFFT_TUNE_ADJOINT), best of `reps` (<= 0: 5), and keeps the fastest.
- * ms_out (may be NULL) receives CUNFFT_TUNE_CONFIGS times in milliseconds,
- * -1 where a configuration does not apply.  Allocates and frees temporaries,
+ * nodes (set them first) in each direction of `directions`
+ * (CUNFFT_TUNE_FORWARD and/or CUNFFT_TUNE_ADJOINT), best of `reps` (<= 0: 5),
+ * and keeps the fastest per direction (the two directions may differ; each
+ * then has its own node binning).  ms_out (may be NULL) receives
+ * 2 * CUNFFT_TUNE_CONFIGS times in milliseconds, the forward times of the
+ * configurations followed by the adjoint ones, -1 where a configuration does
+ * not apply or a direction was not tuned.  Allocates and frees temporaries,
  * runs on `stream` and synchronizes it. */
 int cunfft_autotune(cunfft_plan *plan, int directions, int reps, float *ms_out, void *stream);
-/* The plan's gridding configuration (-1 for an uninitialised plan). */
-int cunfft_get_config(const cunfft_plan *plan);
-/* Select a configuration; the binning is rebuilt by the next transform.
- * Synchronizes the device. */
-int cunfft_set_config(cunfft_plan *plan, int config);
+/* The configuration of one direction (CUNFFT_TUNE_FORWARD or
+ * CUNFFT_TUNE_ADJOINT); -1 for an uninitialised plan or another direction. */
+int cunfft_get_config(const cunfft_plan *plan, int direction);
+/* Select a configuration for the given directions; the binning is rebuilt by
+ * the next transform.  Synchronizes the device.  A configuration that does
+ * not exist (CUNFFT_ERR_INVALID_ARG) or whose LDS tile does not fit the
+ * plan's window (CUNFFT_ERR_UNSUPPORTED) is refused, and the plan stays as
+ * it was. */
+int cunfft_set_config(cunfft_plan *plan, int directions, int config);
 /* Direct NDFT on the GPU in FP64 for the listed outputs only (rows of f or
  * entries of fhat by flat index); rows == NULL means all outputs. */
 int cundft_transform_rows_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
@@ -317,8 +324,8 @@ int cunfftf_backward_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                          const cunfftFloatComplex *grad_f_dev, cunfftFloatComplex *grad_fhat_dev,
                          float *grad_x_dev, void *stream);
 int cunfftf_autotune(cunfftf_plan *plan, int directions, int reps, float *ms_out, void *stream);
-int cunfftf_get_config(const cunfftf_plan *plan);
-int cunfftf_set_config(cunfftf_plan *plan, int config);
+int cunfftf_get_config(const cunfftf_plan *plan, int direction);
+int cunfftf_set_config(cunfftf_plan *plan, int directions, int config);
 int cunfftf_ndft_transform_rows_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                                     const long long *rows_dev, long long n_rows,
                                     cunfftFloatComplex *out_dev, void *stream);

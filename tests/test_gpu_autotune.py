@@ -60,7 +60,7 @@ def test_every_configuration_matches_the_oracle(gpu, orc, precision, kind, case)
     ref_h = orc.nfft_adjoint(N, n, m, x, q(fv, precision))
     tol = 1e-12 if precision == "double" else 1e-5
     with gpu.Plan(N, M, n=n, m=m, precision=precision) as p:
-        assert p.config == 0
+        assert p.config == (0, 0)
         for c in range(gpu.TUNE_CONFIGS):
             try:
                 p.config = c
@@ -69,9 +69,9 @@ def test_every_configuration_matches_the_oracle(gpu, orc, precision, kind, case)
                 # the plan keeps its configuration and stays usable
                 print(N, precision, "config", c, "refused:", e)
                 assert e.status == 2 and 0 < c < gpu.TILE_SHAPES
-                assert p.config == c - 1
+                assert p.config == (c - 1, c - 1)
                 continue
-            assert p.config == c
+            assert p.config == (c, c)
             f, h = fwd_adj(p, x, fh, fv)
             ef, eh = orc.rel_l2(ref_f, f), orc.rel_l2(ref_h, h)
             print(N, kind, precision, "config", c, ef, eh)
@@ -90,12 +90,13 @@ def test_autotune_keeps_the_fastest_configuration(gpu, orc, precision):
     with gpu.Plan(N, M, n=n, m=m, precision=precision) as p:
         p.x[:] = x
         p.copyDataToDevice()
-        chosen, ms = p.autotune(reps=3, stream=torch.cuda.current_stream().cuda_stream)
-        print(precision, "autotune", chosen, ms)
-        assert len(ms) == gpu.TUNE_CONFIGS
-        valid = [c for c in range(gpu.TUNE_CONFIGS) if ms[c] >= 0]
-        assert 0 in valid and gpu.TILE_SHAPES in valid
-        assert chosen in valid and ms[chosen] == min(ms[c] for c in valid)
+        chosen, ms_f, ms_a = p.autotune(reps=3, stream=torch.cuda.current_stream().cuda_stream)
+        print(precision, "autotune", chosen, ms_f, ms_a)
+        assert len(ms_f) == len(ms_a) == gpu.TUNE_CONFIGS
+        for d, ms in enumerate((ms_f, ms_a)):
+            valid = [c for c in range(gpu.TUNE_CONFIGS) if ms[c] >= 0]
+            assert 0 in valid and gpu.TILE_SHAPES in valid
+            assert chosen[d] in valid and ms[chosen[d]] == min(ms[c] for c in valid)
         assert p.config == chosen
         f, h = fwd_adj(p, x, fh, fv)
     tol = 1e-12 if precision == "double" else 1e-5
@@ -103,11 +104,38 @@ def test_autotune_keeps_the_fastest_configuration(gpu, orc, precision):
     assert orc.rel_l2(orc.nfft_adjoint(N, n, m, x, q(fv, precision)), h) <= tol
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_forward_and_adjoint_configurations_can_differ(gpu, orc, precision):
+    """Each direction keeps its own binning when the configurations differ;
+    re-setting the nodes rebuilds both."""
+    N, n, m, M = [32, 32, 32], [64, 64, 64], 4, 30000
+    r = np.random.default_rng(78)
+    tol = 1e-12 if precision == "double" else 1e-5
+    with gpu.Plan(N, M, n=n, m=m, precision=precision) as p:
+        for cf, ca in ((1, gpu.TILE_SHAPES), (gpu.TILE_SHAPES, 0), (0, 1)):
+            p.config = (cf, ca)
+            assert p.config == (cf, ca)
+            for kind in ("uniform", "clustered"):
+                x = nodes(r, M, 3, kind)
+                fh = crandn(r, int(np.prod(N)))
+                fv = crandn(r, M)
+                f, h = fwd_adj(p, x, fh, fv)
+                ef = orc.rel_l2(orc.nfft_trafo(N, n, m, x, q(fh, precision)), f)
+                eh = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, q(fv, precision)), h)
+                print(precision, (cf, ca), kind, ef, eh)
+                assert ef <= tol and eh <= tol
+
+
 def test_one_dimensional_plans_have_one_configuration(gpu):
     with gpu.Plan([256], 1000, n=[512], m=4) as p:
         p.x[:] = np.random.default_rng(1).uniform(-0.5, 0.5, (1000, 1))
         p.copyDataToDevice()
-        chosen, ms = p.autotune(reps=2)
-        assert chosen == 0 and ms[0] >= 0 and all(v == -1 for v in ms[1:])
+        chosen, ms_f, ms_a = p.autotune(reps=2)
+        assert chosen == (0, 0)
+        for ms in (ms_f, ms_a):
+            assert ms[0] >= 0 and all(v == -1 for v in ms[1:])
         with pytest.raises(gpu.CunfftError):
             p.config = 1
+        assert p.config == (0, 0)
+        with pytest.raises(gpu.CunfftError):
+            p.autotune(directions=4)

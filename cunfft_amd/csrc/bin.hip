@@ -252,8 +252,10 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
         if (planes()) {
             desc = dev_alloc((size_t)M * (elem_bytes == 4 ? sizeof(NodeDesc<float>)
                                                           : sizeof(NodeDesc<double>)));
-            slots = dev_alloc((size_t)max_chunks * slot_cells * 2 * elem_bytes);
-            fs = dev_alloc((size_t)M * 2 * elem_bytes);
+            if (own_scratch) {
+                slots = dev_alloc((size_t)max_chunks * slot_cells * 2 * elem_bytes);
+                fs = dev_alloc((size_t)M * 2 * elem_bytes);
+            }
         } else {
             xs = (double *)dev_alloc(sizeof(double) * M * d);
         }
@@ -355,15 +357,14 @@ __device__ __forceinline__ int node_local(const double *__restrict__ xs, uint32_
 
 // --------------------------------------------------------- tiled gather --
 template <typename T, int D, int W>
-__global__ __launch_bounds__(kTileThreads) void k_gather_tiled(
-    const cplx_t<T> *__restrict__ g, const double *__restrict__ xs, const uint32_t *__restrict__ perm,
-    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f,
-    Geom ge, BinGeom bg)
+__device__ __forceinline__ void gather_tiled(const cplx_t<T> *__restrict__ g,
+                                             const double *__restrict__ xs,
+                                             const uint32_t *__restrict__ perm, const Chunk ch,
+                                             cplx_t<T> *__restrict__ f, const Geom &ge,
+                                             const BinGeom &bg)
 {
     using PD = Pad<D, W>;
     __shared__ cplx_t<T> tile[PD::PT];
-    if (blockIdx.x >= n_chunks[0]) return;
-    const Chunk ch = chunks[blockIdx.x];
     int64_t o[3];
     tile_origin<D>(ch.tile, bg, o);
     for (int idx = threadIdx.x; idx < PD::PT; idx += kTileThreads)
@@ -419,6 +420,25 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_tiled(
     }
 }
 
+template <typename T, int D, int W>
+__global__ __launch_bounds__(kTileThreads) void k_gather_tiled(
+    const cplx_t<T> *__restrict__ g, const double *__restrict__ xs, const uint32_t *__restrict__ perm,
+    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f,
+    Geom ge, BinGeom bg)
+{
+    if (blockIdx.x >= n_chunks[0]) return;
+    gather_tiled<T, D, W>(g, xs, perm, chunks[blockIdx.x], f, ge, bg);
+}
+
+template <typename T, int D, int W>
+__global__ __launch_bounds__(kTileThreads) void k_gather_tiled_batch(
+    const GridJob<T> *__restrict__ jobs, const BlockMap *__restrict__ map)
+{
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    gather_tiled<T, D, W>(J.g, J.xs, J.perm, J.chunks[bm.blk], J.f_out, J.ge, J.bg);
+}
+
 // -------------------------------------------------------- tiled scatter --
 template <typename T>
 __device__ __forceinline__ void lds_add(T *p, T v)
@@ -427,16 +447,15 @@ __device__ __forceinline__ void lds_add(T *p, T v)
 }
 
 template <typename T, typename A, int D, int W>
-__global__ __launch_bounds__(kTileThreads) void k_scatter_tiled(
-    const double *__restrict__ xs, const uint32_t *__restrict__ perm, const cplx_t<T> *__restrict__ f,
-    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ g,
-    Geom ge, BinGeom bg)
+__device__ __forceinline__ void scatter_tiled(const double *__restrict__ xs,
+                                              const uint32_t *__restrict__ perm,
+                                              const cplx_t<T> *__restrict__ f, const Chunk ch,
+                                              cplx_t<T> *__restrict__ g, const Geom &ge,
+                                              const BinGeom &bg)
 {
     using PD = Pad<D, W>;
     __shared__ A acc_re[PD::PT];
     __shared__ A acc_im[PD::PT];
-    if (blockIdx.x >= n_chunks[0]) return;
-    const Chunk ch = chunks[blockIdx.x];
     int64_t o[3];
     tile_origin<D>(ch.tile, bg, o);
     for (int idx = threadIdx.x; idx < PD::PT; idx += kTileThreads) {
@@ -495,10 +514,42 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter_tiled(
     }
 }
 
+template <typename T, typename A, int D, int W>
+__global__ __launch_bounds__(kTileThreads) void k_scatter_tiled(
+    const double *__restrict__ xs, const uint32_t *__restrict__ perm, const cplx_t<T> *__restrict__ f,
+    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ g,
+    Geom ge, BinGeom bg)
+{
+    if (blockIdx.x >= n_chunks[0]) return;
+    scatter_tiled<T, A, D, W>(xs, perm, f, chunks[blockIdx.x], g, ge, bg);
+}
+
+template <typename T, typename A, int D, int W>
+__global__ __launch_bounds__(kTileThreads) void k_scatter_tiled_batch(
+    const GridJob<T> *__restrict__ jobs, const BlockMap *__restrict__ map)
+{
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    scatter_tiled<T, A, D, W>(J.xs, J.perm, J.f_in, J.chunks[bm.blk], J.g, J.ge, J.bg);
+}
+
+// Grid <- 0 for the node-lane scatter of every job, 1024 cells per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_batch(const GridJob<T> *__restrict__ jobs,
+                                                    const BlockMap *__restrict__ map)
+{
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    const int64_t c0 = (int64_t)bm.blk * 1024 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (c0 + 256 * k < J.n_total) J.g[c0 + 256 * k] = cmake<T>(0, 0);
+}
+
 // ------------------------------------------------------------ dispatch --
 template <typename T, int D, int W>
-static void gather_tiled(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
-                         hipStream_t s)
+static void launch_gather_lanes(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,
+                                hipStream_t s)
 {
     if constexpr (tile_fits<T, D, W>()) {
         k_gather_tiled<T, D, W><<<(unsigned)b.max_chunks, kTileThreads, 0, s>>>(
@@ -509,8 +560,8 @@ static void gather_tiled(const Geom &ge, const Binning &b, const cplx_t<T> *g, c
 }
 
 template <typename T, int D, int W>
-static void scatter_tiled(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
-                          hipStream_t s)
+static void launch_scatter_lanes(const Geom &ge, const Binning &b, const cplx_t<T> *f,
+                                 cplx_t<T> *g, hipStream_t s)
 {
     if constexpr (tile_fits<T, D, W>()) {
         k_scatter_tiled<T, double, D, W><<<(unsigned)b.max_chunks, kTileThreads, 0, s>>>(
@@ -548,9 +599,9 @@ void launch_gather_binned(const Geom &ge, const Binning &b, const cplx_t<T> *g, 
     const cplx_t<T> *a0 = g;
     cplx_t<T> *a1 = f;
     switch (ge.d) {
-    case 1: CUNFFT_TILED_W(gather_tiled, 1) break;
-    case 2: CUNFFT_TILED_W(gather_tiled, 2) break;
-    default: CUNFFT_TILED_W(gather_tiled, 3) break;
+    case 1: CUNFFT_TILED_W(launch_gather_lanes, 1) break;
+    case 2: CUNFFT_TILED_W(launch_gather_lanes, 2) break;
+    default: CUNFFT_TILED_W(launch_gather_lanes, 3) break;
     }
     CUNFFT_HIP(hipGetLastError());
 }
@@ -573,13 +624,76 @@ bool launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, co
     const cplx_t<T> *a0 = f;
     cplx_t<T> *a1 = g;
     switch (ge.d) {
-    case 1: CUNFFT_TILED_W(scatter_tiled, 1) break;
-    case 2: CUNFFT_TILED_W(scatter_tiled, 2) break;
-    default: CUNFFT_TILED_W(scatter_tiled, 3) break;
+    case 1: CUNFFT_TILED_W(launch_scatter_lanes, 1) break;
+    case 2: CUNFFT_TILED_W(launch_scatter_lanes, 2) break;
+    default: CUNFFT_TILED_W(launch_scatter_lanes, 3) break;
     }
     CUNFFT_HIP(hipGetLastError());
     return false;
 }
+
+// ------------------------------------------------------------ batches --
+int tile_fold_slices(int d, int variant)
+{
+    int64_t T[3];
+    tile_shape(d, variant, T);
+    const int64_t core = T[0] * T[1] * T[2];
+    return core >= 512 ? (int)(core / 512) : 1;
+}
+
+template <typename T, int D, int W>
+static void launch_lanes_batch(BatchStage st, const GridJob<T> *jobs, const BlockMap *map,
+                               uint32_t nblk, hipStream_t s)
+{
+    if constexpr (tile_fits<T, D, W>()) {
+        if (st == BS_GATHER)
+            k_gather_tiled_batch<T, D, W><<<nblk, kTileThreads, 0, s>>>(jobs, map);
+        else
+            k_scatter_tiled_batch<T, double, D, W><<<nblk, kTileThreads, 0, s>>>(jobs, map);
+    } else {
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "tile does not fit in LDS");
+    }
+}
+
+template <typename T>
+void launch_batch_stage(BatchStage st, const GridJob<T> *jobs, const BlockMap *map, uint32_t nblk,
+                        int d, int W, int config, hipStream_t s)
+{
+    if (nblk == 0) return;
+    if (st == BS_ZERO) {
+        k_zero_batch<T><<<nblk, 256, 0, s>>>(jobs, map);
+    } else if ((st == BS_GATHER && (d == 1 || config == kTileVariants)) || st == BS_SCATTER_LANES) {
+        const BatchStage a0 = st;
+        const BlockMap *a1 = map;
+#define CUNFFT_LANES_W(D)                                                            \
+    switch (W) {                                                                     \
+    case 4: launch_lanes_batch<T, D, 4>(a0, jobs, a1, nblk, s); break;               \
+    case 6: launch_lanes_batch<T, D, 6>(a0, jobs, a1, nblk, s); break;               \
+    case 8: launch_lanes_batch<T, D, 8>(a0, jobs, a1, nblk, s); break;               \
+    case 10: launch_lanes_batch<T, D, 10>(a0, jobs, a1, nblk, s); break;             \
+    case 12: launch_lanes_batch<T, D, 12>(a0, jobs, a1, nblk, s); break;             \
+    case 14: launch_lanes_batch<T, D, 14>(a0, jobs, a1, nblk, s); break;             \
+    case 16: if constexpr (D < 3) launch_lanes_batch<T, D, 16>(a0, jobs, a1, nblk, s); break; \
+    case 18: if constexpr (D < 3) launch_lanes_batch<T, D, 18>(a0, jobs, a1, nblk, s); break; \
+    default: throw Error(CUNFFT_ERR_UNSUPPORTED, "tiled gridding needs m <= 8");     \
+    }
+        switch (d) {
+        case 1: CUNFFT_LANES_W(1) break;
+        case 2: CUNFFT_LANES_W(2) break;
+        default: CUNFFT_LANES_W(3) break;
+        }
+#undef CUNFFT_LANES_W
+    } else {
+        launch_tile_batch<T>(st, jobs, map, nblk, d, W, config, s);
+        return;
+    }
+    CUNFFT_HIP(hipGetLastError());
+}
+
+template void launch_batch_stage<float>(BatchStage, const GridJob<float> *, const BlockMap *,
+                                        uint32_t, int, int, int, hipStream_t);
+template void launch_batch_stage<double>(BatchStage, const GridJob<double> *, const BlockMap *,
+                                         uint32_t, int, int, int, hipStream_t);
 
 template void launch_gather_binned<float>(const Geom &, const Binning &, const float2 *,
                                           const double *, float2 *, hipStream_t);

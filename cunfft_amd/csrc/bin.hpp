@@ -86,6 +86,7 @@ struct Binning {
     void *slots = nullptr;          // max_chunks padded tiles (adjoint partial sums)
     void *fs = nullptr;             // M: adjoint coefficients in sorted order
     int elem_bytes = 8;             // bytes of one real component (4: FP32 plan, 8: FP64)
+    bool own_scratch = true;        // allocate slots and fs (a batch brings its own)
 
     // Decide tile shape for this geometry; false if the tiled path does not apply.
     bool plan(const Geom &ge, int elem_bytes, int kernel_variant, int tile_variant = 0);
@@ -101,6 +102,58 @@ struct BinGeom {
     int64_t ntiles;
 };
 BinGeom bin_geom(const Binning &b);
+
+// Window constants of the plane-tiled kernels.
+template <typename T> struct WinConst {
+    T kb[3];    // FP32: log2(e) / b_t (for exp2); FP64: 1 / b_t (for exp)
+    T nrm[3];   // (pi b_t)^-1/2
+    int m;
+};
+template <typename T> WinConst<T> win_const(const Geom &ge);
+
+// ------------------------------------------------------------ batches --
+// One (problem, right-hand side) pair of a batched launch: the problem's grid
+// geometry and node binning, shared by its right-hand sides, and the pair's
+// own buffers.
+template <typename T> struct GridJob {
+    const void *desc;              // NodeDesc<T> by sorted position (plane-tiled kernels)
+    const double *xs;              // nodes by sorted position (node-lane kernels)
+    const uint32_t *perm;          // node at each sorted position
+    const Chunk *chunks;
+    const uint32_t *tile_count;
+    const uint32_t *chunk_start;
+    cplx_t<T> *g;                  // oversampled grid
+    cplx_t<T> *fs;                 // adjoint coefficients by sorted position
+    cplx_t<T> *slots;              // adjoint partial tiles, one per chunk
+    const cplx_t<T> *f_in;         // adjoint input, M coefficients
+    cplx_t<T> *f_out;              // forward output, M values
+    int64_t n_total;               // grid cells
+    Geom ge;
+    BinGeom bg;
+    WinConst<T> wc;
+};
+
+// Stages of a batched transform; each is one launch over all jobs.
+enum BatchStage : int {
+    BS_GATHER = 0,      // forward gridding (blocks: the binning's chunks)
+    BS_COEFFS,          // adjoint coefficients into sorted order (blocks of 256 nodes)
+    BS_SPREAD,          // plane-tiled adjoint accumulation into slots (chunks)
+    BS_FOLD,            // slots -> grid (tiles x fold slices)
+    BS_ZERO,            // grid <- 0 for the node-lane scatter (blocks of 1024 cells)
+    BS_SCATTER_LANES,   // node-lane adjoint accumulation (chunks)
+    BS_COUNT
+};
+// Blocks of job-independent size in the fold: slices of 512 core cells.
+int tile_fold_slices(int d, int variant);
+// One launch of stage st over the jobs of a batch whose problems share the
+// dimension d, window width W, precision T and kernel configuration
+// (plane-tiled variant 0..kTileVariants-1, or kTileVariants for node lanes).
+template <typename T>
+void launch_batch_stage(BatchStage st, const GridJob<T> *jobs, const BlockMap *map, uint32_t nblk,
+                        int d, int W, int config, hipStream_t s);
+template <typename T>
+void launch_tile_batch(BatchStage st, const GridJob<T> *jobs, const BlockMap *map, uint32_t nblk,
+                       int d, int W, int variant, hipStream_t s);
 
 template <typename T>
 void launch_gather_binned(const Geom &ge, const Binning &b, const cplx_t<T> *g, const double *x,

@@ -14,6 +14,8 @@
 // entries that survive the crop; the last one writes fhat with exp(fac k^2).
 // Twiddles come from a per-length table computed in extended precision on
 // the host.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace cunfft {
@@ -158,7 +160,7 @@ __device__ __forceinline__ void stage2_last(cplx_t<T> *buf, const cplx_t<T> *__r
 }
 
 template <typename T>
-__global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
+__device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
 {
     using C2 = cplx_t<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -173,12 +175,12 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
     const bool rows = (P.S == 1);
     int64_t pa0, s0;
     if (rows) {
-        pa0 = (int64_t)blockIdx.x << logC;
+        pa0 = blk << logC;
         s0 = 0;
     } else {
         const int64_t nch = P.S >> logC;
-        pa0 = (int64_t)blockIdx.x / nch;
-        s0 = ((int64_t)blockIdx.x - pa0 * nch) << logC;
+        pa0 = blk / nch;
+        s0 = (blk - pa0 * nch) << logC;
     }
     if (tid < (rows ? C : 1)) {
         const LineInfo li = line_info<T>(P, pa0 + tid);
@@ -257,6 +259,22 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
 }
 
 template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
+{
+    fft_pass<T>(P, blockIdx.x);
+}
+
+// Passes of many transforms in one grid (a batch's jobs; LDS sized for the
+// largest).
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_pass_batch(const FftPass<T> *__restrict__ passes,
+                                                                const BlockMap *__restrict__ map)
+{
+    const BlockMap bm = map[blockIdx.x];
+    fft_pass<T>(passes[bm.job], bm.blk);
+}
+
+template <typename T>
 void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipStream_t s)
 {
     if (blocks <= 0) return;
@@ -264,7 +282,95 @@ void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipS
     CUNFFT_HIP(hipGetLastError());
 }
 
+template <typename T>
+void launch_fft_batch(const FftPass<T> *passes, const BlockMap *map, uint32_t nblk,
+                      size_t lds_bytes, hipStream_t s)
+{
+    if (nblk == 0) return;
+    k_fft_pass_batch<T><<<nblk, kFftThreads, lds_bytes, s>>>(passes, map);
+    CUNFFT_HIP(hipGetLastError());
+}
+
+static int ilog2_i(int64_t v)
+{
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+
+static int64_t next_pow2_i(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Lines per workgroup: rows (S = 1) are packed up to 2048 elements per
+// workgroup; strided passes take C adjacent columns (coalesced C-wide reads)
+// up to 4096 elements.
+template <typename T>
+FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *roc,
+                            const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
+                            cplx_t<T> *fhat_out, cplx_t<T> *g)
+{
+    const int d = ge.d;
+    FftLaunch<T> out{};
+    FftPass<T> &P = out.P;
+    P.g = g;
+    P.fhat_in = fhat_in;
+    P.fhat_out = fhat_out;
+    P.tw = tw;
+    P.roc = roc;
+    for (int q = 0; q < 3; ++q) {
+        P.rocoff[q] = rocoff[q];
+        P.n[q] = ge.n[q];
+        P.N[q] = ge.N[q];
+    }
+    P.d = d;
+    P.t = t;
+    P.L = (int)ge.n[t];
+    P.logL = ilog2_i(P.L);
+    P.S = 1;
+    for (int q = t + 1; q < d; ++q) P.S *= ge.n[q];
+    P.n_pa = 1;
+    for (int q = 0; q < t; ++q) P.n_pa *= ge.N[q];
+    P.sign = forward ? -1 : +1;
+    if (forward)
+        P.mode = (t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED;
+    else
+        P.mode = (t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED;
+    const int64_t target = 2048;
+    int64_t C;
+    if (P.S == 1) {
+        C = P.L >= target ? 1 : target / P.L;
+        C = std::min<int64_t>(C, 64);
+        C = std::min<int64_t>(C, next_pow2_i(P.n_pa));
+        out.blocks = (P.n_pa + C - 1) / C;
+        P.ldsL = P.L;
+    } else {
+        const int64_t cdef = sizeof(T) == 8 ? 8 : 16;
+        C = std::min<int64_t>(cdef, P.S);
+        C = std::min<int64_t>(C, std::max<int64_t>(1, 4096 / P.L));
+        out.blocks = P.n_pa * (P.S / C);
+        P.ldsL = C > 1 ? P.L + 1 : P.L;
+    }
+    P.C = (int)C;
+    P.logC = ilog2_i(C);
+    out.lds = (size_t)C * P.ldsL * sizeof(cplx_t<T>);
+    return out;
+}
+
 template void launch_fft_pass<float>(const FftPass<float> &, int64_t, size_t, hipStream_t);
 template void launch_fft_pass<double>(const FftPass<double> &, int64_t, size_t, hipStream_t);
+template void launch_fft_batch<float>(const FftPass<float> *, const BlockMap *, uint32_t, size_t,
+                                      hipStream_t);
+template void launch_fft_batch<double>(const FftPass<double> *, const BlockMap *, uint32_t, size_t,
+                                       hipStream_t);
+template FftLaunch<float> fft_pass_setup<float>(const Geom &, const float2 *, const double *,
+                                                const int64_t *, int, bool, const float2 *,
+                                                float2 *, float2 *);
+template FftLaunch<double> fft_pass_setup<double>(const Geom &, const double2 *, const double *,
+                                                  const int64_t *, int, bool, const double2 *,
+                                                  double2 *, double2 *);
 
 }  // namespace cunfft

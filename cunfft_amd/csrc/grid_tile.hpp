@@ -172,11 +172,6 @@ template <typename T, int D, int W, int PS, int V> struct TileLayout {
 };
 
 // ----------------------------------------------------------- helpers ----
-template <typename T> struct WinConst {
-    T kb[3];    // FP32: log2(e) / b_t (for exp2); FP64: 1 / b_t (for exp)
-    T nrm[3];   // (pi b_t)^-1/2
-    int m;
-};
 
 template <typename T> __device__ __forceinline__ T wexp(T v);
 template <> __device__ __forceinline__ float wexp<float>(float v)
@@ -345,11 +340,11 @@ __device__ __forceinline__ double rlane_t(double v, int k)
 // row are summed with DPP, and the four row sums of each node are added by
 // the node's lane after the batch.
 template <typename T, int D, int W, int NW, int V>
-__global__ __launch_bounds__(NW * 64) void k_tile_gather(
-    const cplx_t<T> *__restrict__ g, const NodeDesc<T> *__restrict__ desc,
-    const uint32_t *__restrict__ perm, const Chunk *__restrict__ chunks,
-    const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f, Geom ge, BinGeom bg,
-    WinConst<T> wc)
+__device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
+                                            const NodeDesc<T> *__restrict__ desc,
+                                            const uint32_t *__restrict__ perm, const Chunk ch,
+                                            cplx_t<T> *__restrict__ f, const Geom &ge,
+                                            const BinGeom &bg, const WinConst<T> &wc)
 {
     using L = TileLayout<T, D, W, 1, V>;
     using VT = vec16_t<T>;
@@ -362,8 +357,6 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
     __shared__ __align__(16) C tile[L::CELLS];
     __shared__ WinBatch<T, D, W, NB> wbs[NW];
     __shared__ C red[NW][NB][4];
-    if (blockIdx.x >= n_chunks[0]) return;
-    const Chunk ch = chunks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nbat = (ch.count + NB - 1) / NB;
 
@@ -498,6 +491,28 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
     }
 }
 
+template <typename T, int D, int W, int NW, int V>
+__global__ __launch_bounds__(NW * 64) void k_tile_gather(
+    const cplx_t<T> *__restrict__ g, const NodeDesc<T> *__restrict__ desc,
+    const uint32_t *__restrict__ perm, const Chunk *__restrict__ chunks,
+    const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f, Geom ge, BinGeom bg,
+    WinConst<T> wc)
+{
+    if (blockIdx.x >= n_chunks[0]) return;
+    tile_gather<T, D, W, NW, V>(g, desc, perm, chunks[blockIdx.x], f, ge, bg, wc);
+}
+
+// The gather of every chunk of every job of a batch in one grid.
+template <typename T, int D, int W, int NW, int V>
+__global__ __launch_bounds__(NW * 64) void k_tile_gather_batch(const GridJob<T> *__restrict__ jobs,
+                                                               const BlockMap *__restrict__ map)
+{
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    tile_gather<T, D, W, NW, V>(J.g, static_cast<const NodeDesc<T> *>(J.desc), J.perm,
+                                J.chunks[bm.blk], J.f_out, J.ge, J.bg, J.wc);
+}
+
 // ------------------------------------------------------------ spread ----
 // NW wavefronts share the tile; wavefront w owns the planes p0 = w (mod NW),
 // so wavefronts never touch the same cell, and every wavefront walks all the
@@ -508,10 +523,9 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
 // reach past the window, the lanes concerned read and write a private dummy
 // unit past the tile.  fs holds the coefficients in sorted node order.
 template <typename T, int D, int W, int NW, int V>
-__global__ __launch_bounds__(NW * 64) void k_tile_spread(
-    const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
-    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks,
-    cplx_t<T> *__restrict__ slots, WinConst<T> wc)
+__device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc,
+                                            const cplx_t<T> *__restrict__ fs, const Chunk ch,
+                                            cplx_t<T> *__restrict__ slot, const WinConst<T> &wc)
 {
     using L = TileLayout<T, D, W, NW, V>;
     using VT = vec16_t<T>;
@@ -524,8 +538,6 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
     constexpr int NB = batch_nodes<T, D, W>(TILE_UNITS * 16, NW, 0);
     __shared__ __align__(16) VT tile[TILE_UNITS];
     __shared__ WinBatch<T, D, W, NB> wbs[NW];
-    if (blockIdx.x >= n_chunks[0]) return;
-    const Chunk ch = chunks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     NodeDesc<T> nd{};
@@ -633,7 +645,6 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
 
     // ---- padded tile -> slot, destination-blocked
     const C *tc = reinterpret_cast<const C *>(tile);
-    C *slot = slots + (int64_t)blockIdx.x * L::SC;
 #pragma unroll
     for (int q0 = 0; q0 < L::Q0; ++q0)
 #pragma unroll
@@ -664,6 +675,29 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
             }
 }
 
+template <typename T, int D, int W, int NW, int V>
+__global__ __launch_bounds__(NW * 64) void k_tile_spread(
+    const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
+    const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks,
+    cplx_t<T> *__restrict__ slots, WinConst<T> wc)
+{
+    using L = TileLayout<T, D, W, NW, V>;
+    if (blockIdx.x >= n_chunks[0]) return;
+    tile_spread<T, D, W, NW, V>(desc, fs, chunks[blockIdx.x], slots + (int64_t)blockIdx.x * L::SC,
+                                wc);
+}
+
+template <typename T, int D, int W, int NW, int V>
+__global__ __launch_bounds__(NW * 64) void k_tile_spread_batch(const GridJob<T> *__restrict__ jobs,
+                                                               const BlockMap *__restrict__ map)
+{
+    using L = TileLayout<T, D, W, NW, V>;
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    tile_spread<T, D, W, NW, V>(static_cast<const NodeDesc<T> *>(J.desc), J.fs, J.chunks[bm.blk],
+                                J.slots + (int64_t)bm.blk * L::SC, J.wc);
+}
+
 // Coefficients in sorted node order for the adjoint: fs[s] = f[perm[s]].
 template <typename T>
 __global__ __launch_bounds__(256) void k_sorted_coeffs(const cplx_t<T> *__restrict__ f,
@@ -672,6 +706,16 @@ __global__ __launch_bounds__(256) void k_sorted_coeffs(const cplx_t<T> *__restri
 {
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s < M) fs[s] = f[perm[s]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sorted_coeffs_batch(const GridJob<T> *__restrict__ jobs,
+                                                             const BlockMap *__restrict__ map)
+{
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    const int64_t s = (int64_t)bm.blk * 256 + threadIdx.x;
+    if (s < J.ge.M) J.fs[s] = J.f_in[J.perm[s]];
 }
 
 // -------------------------------------------------------------- fold ----
@@ -683,20 +727,21 @@ __global__ __launch_bounds__(256) void k_sorted_coeffs(const cplx_t<T> *__restri
 // further chunks follow), and the sums run in a fixed order: the result does
 // not depend on scheduling.
 template <typename T, int D, int W, int PS, int V>
-__global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__ slots,
-                                                   const uint32_t *__restrict__ tile_count,
-                                                   const uint32_t *__restrict__ chunk_start,
-                                                   cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
+__device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
+                                          const uint32_t *__restrict__ tile_count,
+                                          const uint32_t *__restrict__ chunk_start,
+                                          cplx_t<T> *__restrict__ g, const Geom &ge,
+                                          const BinGeom &bg, uint32_t tile, int slice)
 {
     using L = TileLayout<T, D, W, PS, V>;
     using C = cplx_t<T>;
     constexpr int CORE = L::T0 * L::T1 * L::TL;
     constexpr int CPT = CORE >= 512 ? 2 : (CORE + 255) / 256;   // cells per thread per slice
-    const int slice0 = blockIdx.y * 256 * CPT;                    // first core cell of this slice
+    const int slice0 = slice * 256 * CPT;                          // first core cell of this slice
     constexpr int QR = L::Q1 * L::QL;            // candidates per leading-axis offset
     uint32_t tk[3] = {0, 0, 0};
     {
-        uint32_t rem = blockIdx.x;
+        uint32_t rem = tile;
         for (int t = D - 1; t >= 0; --t) {
             tk[t] = rem % (uint32_t)bg.nt[t];
             rem /= (uint32_t)bg.nt[t];
@@ -790,6 +835,29 @@ __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__
         }
 }
 
+template <typename T, int D, int W, int PS, int V>
+__global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__ slots,
+                                                   const uint32_t *__restrict__ tile_count,
+                                                   const uint32_t *__restrict__ chunk_start,
+                                                   cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
+{
+    tile_fold<T, D, W, PS, V>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x, blockIdx.y);
+}
+
+// Batched fold: block blk of a job is tile blk / slices, slice blk % slices.
+template <typename T, int D, int W, int PS, int V>
+__global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__restrict__ jobs,
+                                                         const BlockMap *__restrict__ map)
+{
+    using L = TileLayout<T, D, W, PS, V>;
+    constexpr int CORE = L::T0 * L::T1 * L::TL;
+    constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<T> &J = jobs[bm.job];
+    tile_fold<T, D, W, PS, V>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg,
+                              bm.blk / SLICES, (int)(bm.blk % SLICES));
+}
+
 // ------------------------------------------------------------ dispatch --
 // Wavefronts per workgroup.  A tile small enough for two or more workgroups
 // per CU gets 4 forward wavefronts (node batches) and 2 adjoint wavefronts
@@ -803,7 +871,7 @@ template <typename T, int D, int W, int V> struct Waves {
     static constexpr int spread = alone ? 4 : 2;
 };
 
-template <typename T> static WinConst<T> win_const(const Geom &ge)
+template <typename T> WinConst<T> win_const(const Geom &ge)
 {
     WinConst<T> w{};
     const double sc = sizeof(T) == 4 ? 1.4426950408889634 : 1.0;   // log2(e) for exp2
@@ -928,6 +996,50 @@ template <typename T> struct SpreadOp {
         }
     }
 };
+
+template <typename T> struct BatchOp {
+    BatchStage st;
+    const GridJob<T> *jobs;
+    const BlockMap *map;
+    uint32_t nblk;
+    hipStream_t s;
+    template <int D, int W, int V> bool run()
+    {
+        if constexpr (fits_all<T, D, W, V>()) {
+            constexpr int NWG = Waves<T, D, W, V>::gather;
+            constexpr int NWS = Waves<T, D, W, V>::spread;
+            switch (st) {
+            case BS_GATHER:
+                k_tile_gather_batch<T, D, W, NWG, V><<<nblk, NWG * 64, 0, s>>>(jobs, map);
+                return true;
+            case BS_COEFFS:
+                k_sorted_coeffs_batch<T><<<nblk, 256, 0, s>>>(jobs, map);
+                return true;
+            case BS_SPREAD:
+                k_tile_spread_batch<T, D, W, NWS, V><<<nblk, NWS * 64, 0, s>>>(jobs, map);
+                return true;
+            case BS_FOLD:
+                k_tile_fold_batch<T, D, W, NWS, V><<<nblk, 256, 0, s>>>(jobs, map);
+                return true;
+            default:
+                return false;
+            }
+        } else {
+            return false;
+        }
+    }
+};
+
+template <typename T>
+void launch_tile_batch(BatchStage st, const GridJob<T> *jobs, const BlockMap *map, uint32_t nblk,
+                       int d, int W, int variant, hipStream_t s)
+{
+    if (nblk == 0) return;
+    BatchOp<T> op{st, jobs, map, nblk, s};
+    if (!with_dwv(d, W, variant, op))
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "no batched plane-tiled kernel for this stage");
+    CUNFFT_HIP(hipGetLastError());
+}
 
 template <typename T>
 void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,

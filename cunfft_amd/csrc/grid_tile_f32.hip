@@ -10,5 +10,8 @@ template void launch_tile_gather<float>(const Geom &, const Binning &, const flo
                                        hipStream_t);
 template void launch_tile_spread<float>(const Geom &, const Binning &, const float2 *, float2 *,
                                        hipStream_t, hipEvent_t);
+template WinConst<float> win_const<float>(const Geom &);
+template void launch_tile_batch<float>(BatchStage, const GridJob<float> *, const BlockMap *, uint32_t,
+                                      int, int, int, hipStream_t);
 
 }  // namespace cunfft

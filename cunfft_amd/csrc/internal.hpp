@@ -20,7 +20,23 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define CUNFFT_HIP(call) ::cunfft::hip_check((call), #call)
 
+// Geometry of one transform (window shape from sigma_t = n_t / N_t).
+Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m);
+// Device roll-off tables exp(fac_t k^2) by pruned index, offsets per axis.
+double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3]);
+
 // ----------------------------------------------------------------- FFT ---
+// Target of a cunfftHandle: twiddle tables tw[q] = exp(-2 pi i q / n_t) per
+// axis, computed in extended precision on the host.
+struct FftTables {
+    int prec;
+    int d;
+    int64_t n[3];
+    void *tw[3];
+};
+FftTables *make_fft_tables(int prec, int d, const int64_t *n);
+void free_fft_tables(void *h);
+
 // One Stockham pass along axis t of the row-major grid n_0 x .. x n_{d-1},
 // viewed as [A][L][S] with L = n_t and S = prod_{t'>t} n_t'.  The lines of the
 // pass run over the support (first and last N_t'/2 indices) of the axes
@@ -54,6 +70,29 @@ template <typename T> struct FftPass {
 
 template <typename T>
 void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipStream_t s);
+
+// Pass along axis t of one transform (forward: axes d-1 .. 0, the first from
+// fhat with the roll-off; adjoint: axes 0 .. d-1, the last to fhat), with its
+// launch shape.  tw is the twiddle table of length n_t.
+template <typename T> struct FftLaunch {
+    FftPass<T> P;
+    int64_t blocks;
+    size_t lds;
+};
+template <typename T>
+FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *roc,
+                            const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
+                            cplx_t<T> *fhat_out, cplx_t<T> *g);
+
+// Workgroup b of a batched launch works on block map[b].blk of job map[b].job.
+struct BlockMap {
+    uint32_t job;
+    uint32_t blk;
+};
+// One launch of the passes of many transforms (one FftPass per job).
+template <typename T>
+void launch_fft_batch(const FftPass<T> *passes, const BlockMap *map, uint32_t nblk,
+                      size_t lds_bytes, hipStream_t s);
 
 // ------------------------------------------------------------ gridding ---
 template <typename T>

@@ -47,15 +47,7 @@ template <> struct Traits<float> {
 };
 
 // ------------------------------------------------------------ FFT tables --
-// Target of a cunfftHandle: twiddle tables tw[q] = exp(-2 pi i q / n_t) per axis.
-struct FftTables {
-    int prec;
-    int d;
-    int64_t n[3];
-    void *tw[3];
-};
-
-static FftTables *make_fft_tables(int prec, int d, const int64_t *n)
+FftTables *make_fft_tables(int prec, int d, const int64_t *n)
 {
     FftTables *ft = new FftTables();
     ft->prec = prec;
@@ -98,7 +90,7 @@ static FftTables *make_fft_tables(int prec, int d, const int64_t *n)
     return ft;
 }
 
-static void free_fft_tables(cunfftHandle h)
+void free_fft_tables(void *h)
 {
     FftTables *ft = static_cast<FftTables *>(h);
     if (!ft) return;
@@ -198,6 +190,56 @@ template <typename T> static void init_device_para(typename Traits<T>::plan *p)
     if (p->impl) static_cast<Impl<T> *>(p->impl)->nodes_changed();
 }
 
+// Window shape b_t = 2 sigma_t / (2 sigma_t - 1) * m / pi with sigma_t = n_t / N_t
+// (src/cunfft.cu:351-357) and the gridding geometry of one transform.
+Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m)
+{
+    Geom ge{};
+    ge.d = d;
+    ge.m = m;
+    ge.W = 2 * m + 2;
+    ge.M = M;
+    for (int t = 0; t < 3; ++t) {
+        ge.N[t] = t < d ? N[t] : 1;
+        ge.n[t] = t < d ? n[t] : 1;
+        if (t < d) {
+            const double sigma = (double)n[t] / (double)N[t];
+            const double b = (2.0 * sigma) / (2.0 * sigma - 1.0) * ((double)m / M_PI);
+            ge.invb[t] = 1.0 / b;
+            ge.nrm[t] = 1.0 / sqrt(M_PI * b);
+        } else {
+            ge.invb[t] = ge.nrm[t] = 0.0;
+        }
+    }
+    return ge;
+}
+
+// Device roll-off tables by pruned index q < N_t: exp(fac_t k^2) with
+// k = q or q - N_t and fac_t = pi^2 b_t / n_t^2 (src/cunfft.cu:79-90).
+double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3])
+{
+    std::vector<double> roc;
+    for (int t = 0; t < 3; ++t) rocoff[t] = 0;
+    for (int t = 0; t < ge.d; ++t) {
+        rocoff[t] = (int64_t)roc.size();
+        const double sigma = (double)ge.n[t] / (double)ge.N[t];
+        const double b = (2.0 * sigma) / (2.0 * sigma - 1.0) * ((double)ge.m / M_PI);
+        const double fac = M_PI * M_PI * b / ((double)ge.n[t] * (double)ge.n[t]);
+        const int64_t Nn = ge.N[t];
+        for (int64_t q = 0; q < Nn; ++q) {
+            const int64_t k = q < Nn / 2 ? q : q - Nn;
+            roc.push_back(exp(fac * (double)k * (double)k));
+        }
+    }
+    double *dev = nullptr;
+    CUNFFT_HIP(hipMalloc((void **)&dev, sizeof(double) * roc.size()));
+    const hipError_t e =
+        hipMemcpy(dev, roc.data(), sizeof(double) * roc.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) (void)hipFree(dev);
+    CUNFFT_HIP(e);
+    return dev;
+}
+
 // src/cunfft.cu:60-76: exp(-l^2 / b) for l = 0 .. 2m+1 (evaluated directly).
 static void fg_exp_l(double *out, double b, int offset, int m)
 {
@@ -277,32 +319,15 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
     CUNFFT_HIP(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
     for (int k = 0; k < 4; ++k) CUNFFT_HIP(hipEventCreate(&im->ev[k]));
 
-    Geom &ge = im->geom;
-    ge.d = d;
-    ge.m = p->m;
-    ge.W = 2 * p->m + 2;
-    ge.M = p->M_total;
-    for (int t = 0; t < 3; ++t) {
-        ge.N[t] = t < d ? p->N[t] : 1;
-        ge.n[t] = t < d ? p->n[t] : 1;
-        ge.invb[t] = t < d ? 1.0 / p->b[t] : 0.0;
-        ge.nrm[t] = t < d ? 1.0 / sqrt(M_PI * p->b[t]) : 0.0;
+    int64_t NN[3], nn3[3];
+    for (int t = 0; t < d; ++t) {
+        NN[t] = p->N[t];
+        nn3[t] = p->n[t];
     }
+    im->geom = make_geom(d, NN, nn3, p->M_total, p->m);
     const int c0 = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
     plan_configs<T>(im, c0, c0);
-    // roll-off tables by pruned index p: exp(fac_t k^2), k = p or p - N_t
-    std::vector<double> roc;
-    for (int t = 0; t < d; ++t) {
-        im->rocoff[t] = (int64_t)roc.size();
-        const double fac = M_PI * M_PI * p->b[t] / ((double)p->n[t] * (double)p->n[t]);
-        const int64_t Nn = p->N[t];
-        for (int64_t q = 0; q < Nn; ++q) {
-            const int64_t k = q < Nn / 2 ? q : q - Nn;
-            roc.push_back(exp(fac * (double)k * (double)k));
-        }
-    }
-    CUNFFT_HIP(hipMalloc((void **)&im->roc, sizeof(double) * roc.size()));
-    CUNFFT_HIP(hipMemcpy(im->roc, roc.data(), sizeof(double) * roc.size(), hipMemcpyHostToDevice));
+    im->roc = make_rolloff(im->geom, im->rocoff);
     init_device_para<T>(p);
     int64_t nn[3];
     for (int t = 0; t < d; ++t) nn[t] = p->n[t];
@@ -355,50 +380,10 @@ static void fft_passes(typename Traits<T>::plan *p, Impl<T> *im, FftTables *ft, 
     const int d = p->d;
     for (int k = 0; k < d; ++k) {
         const int t = forward ? d - 1 - k : k;
-        FftPass<T> P{};
-        P.g = g;
-        P.fhat_in = fhat_in;
-        P.fhat_out = fhat_out;
-        P.tw = static_cast<const cplx_t<T> *>(ft->tw[t]);
-        P.roc = im->roc;
-        for (int q = 0; q < 3; ++q) {
-            P.rocoff[q] = im->rocoff[q];
-            P.n[q] = im->geom.n[q];
-            P.N[q] = im->geom.N[q];
-        }
-        P.d = d;
-        P.t = t;
-        P.L = (int)p->n[t];
-        P.logL = ilog2(P.L);
-        P.S = 1;
-        for (int q = t + 1; q < d; ++q) P.S *= p->n[q];
-        P.n_pa = 1;
-        for (int q = 0; q < t; ++q) P.n_pa *= p->N[q];
-        P.sign = forward ? -1 : +1;
-        if (forward)
-            P.mode = (t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED;
-        else
-            P.mode = (t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED;
-        const int64_t target = 2048;
-        int64_t C;
-        int64_t blocks;
-        if (P.S == 1) {
-            C = P.L >= target ? 1 : target / P.L;
-            C = std::min<int64_t>(C, 64);
-            C = std::min<int64_t>(C, next_pow2(P.n_pa));
-            blocks = (P.n_pa + C - 1) / C;
-            P.ldsL = P.L;
-        } else {
-            const int64_t cdef = sizeof(T) == 8 ? 8 : 16;
-            C = std::min<int64_t>(cdef, P.S);
-            C = std::min<int64_t>(C, std::max<int64_t>(1, 4096 / P.L));
-            blocks = P.n_pa * (P.S / C);
-            P.ldsL = C > 1 ? P.L + 1 : P.L;
-        }
-        P.C = (int)C;
-        P.logC = ilog2(C);
-        const size_t lds = (size_t)C * P.ldsL * sizeof(cplx_t<T>);
-        launch_fft_pass<T>(P, blocks, lds, s);
+        const FftLaunch<T> L = fft_pass_setup<T>(im->geom, static_cast<const cplx_t<T> *>(ft->tw[t]),
+                                                 im->roc, im->rocoff, t, forward, fhat_in,
+                                                 fhat_out, g);
+        launch_fft_pass<T>(L.P, L.blocks, L.lds, s);
     }
 }
 
@@ -591,6 +576,8 @@ static void report(const char *what)
     g_last_message = what;
     if (getenv("CUNFFT_VERBOSE")) fprintf(stderr, "cunfft: %s\n", what);
 }
+
+void report_error(const char *what) { report(what); }
 
 template <typename P, typename F> static int guard(P *p, F &&fn)
 {

@@ -285,6 +285,47 @@ int cundft_adjoint_cols_dev(cunfft_plan *plan, const cunfftDoubleComplex *f_dev,
                             const long long *cols_dev, long long n_cols,
                             cunfftDoubleComplex *out_dev, void *stream);
 
+/* ---- batches (no counterpart in the reference) ----
+ * Many uneven problems and right-hand sides in one graph launch: P problems
+ * of one dimension, cut-off m and precision, problem i with its own N_i, n_i
+ * and M_i nodes, and R right-hand sides each.  Every stage of the transform
+ * is one launch over all P*R (problem, right-hand side) pairs; the stage
+ * launches are captured into a hipGraph once and replayed per call.  Each
+ * problem is binned once per node set, for all its right-hand sides.  This
+ * replaces a loop of cunfft_transform / cunfft_adjoint calls over single
+ * plans (src/cunfft.cu:196-224). */
+typedef struct cunfft_batch cunfft_batch;
+/* precision 64 (FP64) or 32 (FP32); N and n hold dim entries per problem
+ * (n == NULL: n_t = 2 * next_pow2(N_t) as cunfft_init, src/cunfft.cu:296-299);
+ * M one per problem.  All device buffers are allocated here.  Every problem
+ * must suit the tiled gridding kernels (n_t >= 256 in 1D, 32 in 2D, 8 in 3D;
+ * m <= 8), else CUNFFT_ERR_UNSUPPORTED. */
+int cunfft_batch_create(cunfft_batch **batch, int precision, int dim, int n_problems,
+                        const uint_t *N, const uint_t *n, const uint_t *M, int m, int n_rhs);
+void cunfft_batch_destroy(cunfft_batch *batch);
+/* Nodes of one problem (d*M_i doubles, device); binned by the next execute. */
+int cunfft_batch_set_nodes_dev(cunfft_batch *batch, int problem, const double *x_dev,
+                               void *stream);
+/* The batch's device buffers of one problem, R right-hand sides each, one
+ * after the other: fhat_in (R*N_total, forward input), f_out (R*M, forward
+ * output), f_in (R*M, adjoint input), fhat_out (R*N_total, adjoint output).
+ * Complex of the batch's precision; any output pointer may be NULL. */
+int cunfft_batch_buffers(cunfft_batch *batch, int problem, void **fhat_in, void **f_out,
+                         void **f_in, void **fhat_out);
+/* Forward (CUNFFT_TUNE_FORWARD: f_out = A fhat_in) and/or adjoint
+ * (CUNFFT_TUNE_ADJOINT: fhat_out = A^H f_in) of every problem and right-hand
+ * side, as one graph launch on `stream` (NULL: the default stream).  The
+ * first call after new nodes or a new configuration bins the nodes and
+ * synchronizes `stream` once. */
+int cunfft_batch_execute(cunfft_batch *batch, int directions, void *stream);
+/* Autotuner over the batch as a whole (cunfft_autotune for batches): times
+ * each gridding configuration's graph per direction and keeps the fastest;
+ * ms_out as for cunfft_autotune.  Synchronizes `stream`. */
+int cunfft_batch_autotune(cunfft_batch *batch, int directions, int reps, float *ms_out,
+                          void *stream);
+int cunfft_batch_get_config(const cunfft_batch *batch, int direction);
+int cunfft_batch_set_config(cunfft_batch *batch, int directions, int config);
+
 /* ======================================================= FP32 family ==== */
 
 void cunfftf_copyDataToDevice(cunfftf_plan *plan);

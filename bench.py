@@ -8,6 +8,13 @@ resident in HBM (nodes binned once, before timing; x is fixed across steps).
 The headline runs in FP32 complex with the FP64 index/fraction split and FP64
 roll-off factors; the same steps are timed in full FP64 beside it.
 
+Beside it (the "batched_2d" object): 96 uneven 2D FP32 problems, N_b x N_b
+with N_b in {64, 96, 128, 192, 256}, n_b = 2 next_pow2(N_b), m = 4, M_b
+uniform in [4096, 65536] nodes, 8 right-hand sides each; one step = forward +
+adjoint of every problem and right-hand side, as one graph launch of the
+batched kernels, against a loop of single-plan launches over the same data.
+Both are autotuned before timing.
+
 Prints one JSON line (the last line of stdout).
 """
 from __future__ import annotations
@@ -63,6 +70,11 @@ def time_plan(cu, precision, x, fh, fv, steps, warmup):
     p.copyDataToDevice()
     p.copyDataToDeviceAd()
     dp = p.device_pointers
+    # gridding kernels chosen by the autotuner on these nodes (untimed, plan time)
+    config, ms_f, ms_a = p.autotune(reps=3)
+    torch.cuda.synchronize()
+    p.copyDataToDevice()
+    p.copyDataToDeviceAd()
     # warm-up (also builds the node binning once)
     for _ in range(warmup):
         p.transform_dev(dp["f_hat"], dp["f"])
@@ -103,7 +115,8 @@ def time_plan(cu, precision, x, fh, fv, steps, warmup):
         rec["adj_fft"].append(t["time_FFT"])
     p._p.flags = p._p.flags & ~cu.MEASURE_TIME
     med = {k: 1e3 * float(np.median(v)) for k, v in rec.items()}
-    return p, {"ms_per_step": ms_per_step, "cold_binning_ms_per_step": cold_ms,
+    return p, {"config": list(config), "autotune_fwd_ms": ms_f, "autotune_adj_ms": ms_a,
+               "ms_per_step": ms_per_step, "cold_binning_ms_per_step": cold_ms,
                "fwd_fft_ms": med["fwd_fft"], "fwd_gather_ms": med["fwd_gather"],
                "adj_scatter_ms": med["adj_grid"] - med["adj_fold"], "adj_fold_ms": med["adj_fold"],
                "adj_fft_ms": med["adj_fft"]}
@@ -185,10 +198,126 @@ def cpu_baseline(x, fh, budget_s=15.0):
     rate = terms / elapsed                              # terms per second
     full_terms = 2.0 * M_NODES * int(np.prod(N3))       # forward + adjoint NDFT
     value = M_NODES / (full_terms / rate)
+    # one thread, on a smaller sample
+    rows1 = rows[: max(1, n_rows // max(1, cores))]
+    t0 = time.perf_counter()
+    oracle.ndft_trafo(N3, x, fh, rows=rows1, nthreads=1)
+    e1 = time.perf_counter() - t0
+    value1 = M_NODES / (full_terms / (len(rows1) * int(np.prod(N3)) / e1))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fcpu:
+            for line in fcpu:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": value, "unit": "nodes/s", "cores": int(cores), "kind": "port",
             "sample": f"FP64 direct NDFT (oracle, OpenMP) of {n_rows} sampled output rows of the "
                       f"forward transform ({terms:.3g} terms in {elapsed:.2f} s); fwd+adj rate "
-                      f"extrapolated linearly to the full 2*M*N_total terms"}
+                      f"extrapolated linearly to the full 2*M*N_total terms",
+            "value_1_thread": value1, "sample_1_thread_rows": int(len(rows1)),
+            "cpu_model": model, "nproc": os.cpu_count()}
+
+
+def bench_batch(cu, steps, warmup, seed=5150):
+    """96 uneven 2D FP32 problems x 8 right-hand sides: one graph launch of the
+    batched stage kernels per step, against a loop of single-plan launches."""
+    import torch
+
+    from cunfft_amd.batch import Batch
+
+    r = np.random.default_rng(seed)
+    P, R, m = 96, 8, 4
+    sizes = r.choice([64, 96, 128, 192, 256], P)
+    N = np.stack([sizes, sizes], axis=1)
+    M = r.integers(4096, 65536, P, endpoint=True)
+    dev = torch.device("cuda")
+    b = Batch(N, M, m=m, n_rhs=R, precision="single")
+    xs = []
+    for i in range(P):
+        x = torch.from_numpy(r.uniform(-0.5, 0.5, (int(M[i]), 2))).to(dev)
+        xs.append(x)
+        b.set_nodes_dev(i, x.data_ptr())
+        fhat_in, _, f_in, _ = b.tensors(i)
+        fhat_in.copy_(torch.complex(torch.randn(fhat_in.shape, device=dev),
+                                    torch.randn(fhat_in.shape, device=dev)))
+        f_in.copy_(torch.complex(torch.randn(f_in.shape, device=dev),
+                                 torch.randn(f_in.shape, device=dev)))
+    torch.cuda.synchronize()
+    config, ms_f, ms_a = b.autotune(reps=3)
+    for _ in range(warmup):
+        b.execute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.execute()
+    torch.cuda.synchronize()
+    ms_batch = 1e3 * (time.perf_counter() - t0) / steps
+    # the same problems as single plans, each autotuned, looped per right-hand side
+    plans, ptrs = [], []
+    for i in range(P):
+        p = cu.Plan(list(N[i]), int(M[i]), n=list(b.n[i]), m=m, precision="single",
+                    flags=cu.CUFFT_INIT | cu.CUFFT_BOTH | cu.NO_HOST_BUFFERS)
+        p.set_nodes_dev(xs[i].data_ptr())
+        p.autotune(reps=2)
+        plans.append(p)
+        fi, fo, ai, ao = b.buffers(i)
+        Nt, Mi = int(N[i][0] * N[i][1]), int(M[i])
+        ptrs.append([(fi + 8 * k * Nt, fo + 8 * k * Mi, ai + 8 * k * Mi, ao + 8 * k * Nt)
+                     for k in range(R)])
+    torch.cuda.synchronize()
+
+    def loop_step():
+        for p, pr in zip(plans, ptrs):
+            for fi, fo, ai, ao in pr:
+                p.transform_dev(fi, fo)
+                p.adjoint_dev(ai, ao)
+
+    for _ in range(warmup):
+        loop_step()
+    torch.cuda.synchronize()
+    loop_steps = max(2, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        loop_step()
+    torch.cuda.synchronize()
+    ms_loop = 1e3 * (time.perf_counter() - t0) / loop_steps
+    # accuracy: the batch's FP32 outputs against FP64 single plans on 3 problems
+    b.execute()
+    torch.cuda.synchronize()
+    errs = []
+    for i in (0, P // 2, P - 1):
+        fi, fo, ai, ao = b.tensors(i)
+        with cu.Plan(list(N[i]), int(M[i]), n=list(b.n[i]), m=m, precision="double",
+                     flags=cu.CUFFT_INIT | cu.CUFFT_BOTH | cu.NO_HOST_BUFFERS) as p64:
+            p64.set_nodes_dev(xs[i].data_ptr())
+            f64 = torch.empty(int(M[i]), dtype=torch.complex128, device=dev)
+            h64 = torch.empty(int(N[i][0] * N[i][1]), dtype=torch.complex128, device=dev)
+            src_f = fi[0].to(torch.complex128).contiguous()
+            src_a = ai[0].to(torch.complex128).contiguous()
+            p64.transform_dev(src_f.data_ptr(), f64.data_ptr())
+            p64.adjoint_dev(src_a.data_ptr(), h64.data_ptr())
+            torch.cuda.synchronize()
+            ef = float((fo[0].to(torch.complex128) - f64).norm() / f64.norm())
+            eh = float((ao[0].to(torch.complex128) - h64).norm() / h64.norm())
+            errs.append(max(ef, eh))
+    for p in plans:
+        p.finalize()
+    b.close()
+    nodes = int(M.sum()) * R
+    return {
+        "workload": "96 uneven 2D problems N_b^2, N_b in {64,96,128,192,256}, n_b = "
+                    "2 next_pow2(N_b), m=4, M_b uniform in [4096,65536], 8 right-hand sides, "
+                    "forward+adjoint, FP32",
+        "value": nodes / (ms_batch * 1e-3), "unit": "nodes/s", "ms_per_step": ms_batch,
+        "node_transforms_per_step": nodes, "loop_of_single_plans_ms_per_step": ms_loop,
+        "loop_value": nodes / (ms_loop * 1e-3), "speedup_vs_loop": ms_loop / ms_batch,
+        "batch_config": list(config), "autotune_fwd_ms": ms_f, "autotune_adj_ms": ms_a,
+        "fp32_vs_fp64_plan_rel_l2_max_3_problems": max(errs),
+        "n_b_counts": {int(v): int((sizes == v).sum()) for v in (64, 96, 128, 192, 256)},
+    }
 
 
 def main():
@@ -197,6 +326,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D workload")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched 2D workload (prints its JSON object)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (to scale HBM "
                          "counters) and exit")
@@ -220,6 +352,9 @@ def main():
     import cunfft_amd as cu
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if a.batch_only:
+        print(json.dumps(bench_batch(cu, a.steps, a.warmup)), flush=True)
+        return 0
     log("bench: building inputs")
     x, fh, fv = make_inputs()
     log("bench: timing FP32")
@@ -233,18 +368,28 @@ def main():
     log(f"bench: {acc}")
     p32.finalize()
     p64.finalize()
+    batch = None
+    if not a.no_batch:
+        log("bench: batched 2D workload")
+        batch = bench_batch(cu, a.steps, a.warmup)
+        log(f"bench: {batch}")
 
     value = M_NODES / (t32["ms_per_step"] * 1e-3)
     value64 = M_NODES / (t64["ms_per_step"] * 1e-3)
     # dominant kernel of the FP32 step and its roofline (algorithmic bytes of one
     # gridding launch: the grid once plus the nodes, see grid_bytes)
-    names = {"fwd_gather_ms": "k_tile_gather<float,3,10,4> (forward gridding)",
-             "adj_scatter_ms": "k_tile_spread<float,3,10,2> (adjoint tile accumulation, "
-                               "with the sorted-coefficient copy)"}
+    cf, ca = t32["config"]
+    shapes = ["8x8x8", "8x8x16", "16x16x8"]
+    names = {"fwd_gather_ms": f"k_tile_gather<float,3,10,NW,{cf}> ({shapes[cf]} tiles, forward "
+                              "gridding)" if cf < 3 else "k_gather_tiled<float,3,10> (node lanes)",
+             "adj_scatter_ms": f"k_tile_spread<float,3,10,NW,{ca}> ({shapes[ca]} tiles, adjoint "
+                               "tile accumulation, with the sorted-coefficient copy)"
+                               if ca < 3 else "k_scatter_tiled<float,double,3,10> (node lanes)"}
     kern = max(names, key=lambda k: t32[k])
     ach = grid_bytes(8) / (t32[kern] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": names[kern], "achieved": ach, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(kern),
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": measured_traffic(kern) if (cf, ca) == (0, 0) else None,
             "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
     cpu = None
     if not a.no_cpu_baseline:
@@ -261,7 +406,7 @@ def main():
         "fp64": {"value": value64, "ms_per_step": t64["ms_per_step"],
                  "fp32_speedup_over_fp64": value / value64},
         "steps_fp32_ms": t32, "steps_fp64_ms": t64, "accuracy": acc,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched_2d": batch,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -59,6 +59,7 @@ template <typename T> struct BatchImpl {
         size_t map_off = 0;            // into maps
         size_t pass_off = 0;           // into passes (FFT stages)
         size_t lds = 0;
+        int max_elems = 0;             // largest C * L of an FFT stage
     };
 
     int d = 0, m = 0, W = 0, R = 0;
@@ -320,6 +321,7 @@ template <typename T> struct BatchImpl {
                     ph.push_back(f.P);
                     for (int64_t k = 0; k < f.blocks; ++k) mh.push_back(BlockMap{job, (uint32_t)k});
                     L.lds = std::max(L.lds, f.lds);
+                    L.max_elems = std::max(L.max_elems, f.P.C << f.P.logL);
                 }
             }
             L.nblk = (uint32_t)(mh.size() - L.map_off);
@@ -373,7 +375,8 @@ template <typename T> struct BatchImpl {
     {
         for (const Launch &L : stages[dir]) {
             if (L.fft_t >= 0)
-                launch_fft_batch<T>(passes + L.pass_off, maps + L.map_off, L.nblk, L.lds, s);
+                launch_fft_batch<T>(passes + L.pass_off, maps + L.map_off, L.nblk, L.lds,
+                                    L.max_elems, s);
             else
                 launch_batch_stage<T>(L.st, jobs[L.dir], maps + L.map_off, L.nblk, d, W,
                                       config[dir], s);

@@ -15,15 +15,22 @@
 // Twiddles come from a per-length table computed in extended precision on
 // the host.
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.hpp"
 
 namespace cunfft {
 
 constexpr int kFftThreads = 256;
-constexpr int kFftMaxElems = 4096;  // C * L per workgroup
-constexpr int kMaxB4 = kFftMaxElems / 4 / kFftThreads;
-constexpr int kMaxB2 = kFftMaxElems / 2 / kFftThreads;
+// Elements C * L per workgroup: up to 8192 in FP32 (so a 512-point column
+// pass still reads 16 columns = 128 contiguous bytes per row), 4096 in FP64.
+// The kernels hold E / 1024 radix-4 butterflies per thread in registers and
+// are instantiated for E = 4096 and (FP32) 8192; a launch takes the smaller
+// one that fits its C * L.
+template <typename T> constexpr int fft_max_elems() { return sizeof(T) == 4 ? 4096 : 4096; }
+constexpr int kFftSmallElems = 4096;
+// global loads in flight per thread (FP64: fewer, the registers run short)
+template <typename T> constexpr int fft_load_batch() { return sizeof(T) == 4 ? 8 : 4; }
 constexpr int kMaxRowsPerGroup = 64;
 
 template <typename T>
@@ -73,7 +80,12 @@ __device__ LineInfo line_info(const FftPass<T> &P, int64_t pa)
     int64_t rem = li.valid ? pa : 0;
     int64_t a = 0, gmul = 1, fq = 0, fmul = 1;
     double fac = 1.0;
-    for (int tt = P.t - 1; tt >= 0; --tt) {
+    // axes tt = t-1 .. 0, unrolled so the per-axis fields are read by constant
+    // index (no private-memory copy of the descriptor)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int tt = 1 - k;
+        if (tt >= P.t) continue;
         const int64_t Nn = P.N[tt], hN = Nn >> 1;
         const int64_t p = rem % Nn;
         rem /= Nn;
@@ -86,12 +98,12 @@ __device__ LineInfo line_info(const FftPass<T> &P, int64_t pa)
         fac *= P.roc[P.rocoff[tt] + p];
     }
     li.gbase = a * (int64_t)P.L * P.S;
-    li.fbase = fq * P.N[P.t];
+    li.fbase = fq * (P.t == 0 ? P.N[0] : (P.t == 1 ? P.N[1] : P.N[2]));
     li.fac = fac;
     return li;
 }
 
-template <typename T>
+template <typename T, int E>
 __device__ __forceinline__ void stage4(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int logL,
                                        int C, int ldsL, int logNs, int sign, int tid)
 {
@@ -99,6 +111,7 @@ __device__ __forceinline__ void stage4(cplx_t<T> *buf, const cplx_t<T> *__restri
     const int Q = 1 << logQ;
     const int Ns = 1 << logNs;
     const int nb = C << logQ;
+    constexpr int kMaxB4 = E / 4 / kFftThreads;
     cplx_t<T> v[kMaxB4][4];
 #pragma unroll
     for (int k = 0; k < kMaxB4; ++k) {
@@ -137,13 +150,14 @@ __device__ __forceinline__ void stage4(cplx_t<T> *buf, const cplx_t<T> *__restri
 }
 
 // Final radix-2 stage (Ns = L/2): butterfly j combines j and j + L/2 in place.
-template <typename T>
+template <typename T, int E>
 __device__ __forceinline__ void stage2_last(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw,
                                             int logL, int C, int ldsL, int sign, int tid)
 {
     const int logH = logL - 1;
     const int H = 1 << logH;
     const int nb = C << logH;
+    constexpr int kMaxB2 = E / 2 / kFftThreads;
 #pragma unroll
     for (int k = 0; k < kMaxB2; ++k) {
         const int b = tid + k * kFftThreads;
@@ -159,7 +173,7 @@ __device__ __forceinline__ void stage2_last(cplx_t<T> *buf, const cplx_t<T> *__r
     __syncthreads();
 }
 
-template <typename T>
+template <typename T, int E>
 __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
 {
     using C2 = cplx_t<T>;
@@ -189,48 +203,80 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
         s_fac[tid] = li.fac;
         s_valid[tid] = li.valid;
     }
+    // the twiddle table of the axis, staged in LDS after the lines
+    C2 *twl = buf + C * ldsL;
+    for (int q = tid; q < L; q += kFftThreads) twl[q] = P.tw[q];
     __syncthreads();
 
     const int total = C << logL;
-    const int Nt = (int)P.N[P.t];
+    const int Nt = (int)(P.t == 0 ? P.N[0] : (P.t == 1 ? P.N[1] : P.N[2]));
     const int hN = Nt >> 1;
-    const double *roct = P.roc + P.rocoff[P.t];
+    const double *roct =
+        P.roc + (P.t == 0 ? P.rocoff[0] : (P.t == 1 ? P.rocoff[1] : P.rocoff[2]));
 
-    for (int e = tid; e < total; e += kFftThreads) {
-        int c, i;
-        if (rows) {
-            c = e >> logL;
-            i = e & (L - 1);
-        } else {
-            c = e & (C - 1);
-            i = e >> logC;
-        }
-        const int li = rows ? c : 0;
-        C2 v = cmake<T>(0, 0);
-        if (s_valid[li]) {
-            const bool insupp = (i < hN) || (i >= L - hN);
-            if (P.mode & FFT_FROM_FHAT) {
-                if (insupp) {
+    // loads in batches of kFftLoadBatch per thread, all issued before the
+    // first LDS store (a pass is bound by memory latency otherwise): every
+    // lane loads, from a clamped address where it has nothing to load, and
+    // the value is zeroed after
+    constexpr int kFftLoadBatch = fft_load_batch<T>();
+    auto stage_in = [&](auto from_fhat_tag) {
+        constexpr bool FROM_FHAT = decltype(from_fhat_tag)::value;
+        const bool in_pruned = P.mode & FFT_IN_PRUNED;
+        for (int e0 = tid; e0 < total; e0 += kFftLoadBatch * kFftThreads) {
+            C2 v[kFftLoadBatch];
+            double fr[kFftLoadBatch];   // roll-off factor of a load from fhat
+            int at[kFftLoadBatch];
+            bool ok[kFftLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kFftLoadBatch; ++u) {
+                const int e = e0 + u * kFftThreads;
+                int c, i;
+                if (rows) {
+                    c = e >> logL;
+                    i = e & (L - 1);
+                } else {
+                    c = e & (C - 1);
+                    i = e >> logC;
+                }
+                const bool in = e < total;
+                const int li = rows && in ? c : 0;
+                at[u] = in ? c * ldsL + i : -1;
+                const bool insupp = (i < hN) || (i >= L - hN);
+                if constexpr (FROM_FHAT) {
+                    ok[u] = in && s_valid[li] && insupp;
                     const int p = i < hN ? i : i - (L - Nt);
                     const int q = p < hN ? p + hN : p - hN;
-                    const C2 a = P.fhat_in[s_fbase[li] + q];
-                    const double f = s_fac[li] * roct[p];
-                    v = cmake<T>((T)((double)a.x * f), (T)((double)a.y * f));
+                    const int64_t fb = s_fbase[li];
+                    const double fl = s_fac[li];
+                    v[u] = P.fhat_in[ok[u] ? fb + q : 0];
+                    fr[u] = fl * roct[ok[u] ? p : 0];
+                } else {
+                    ok[u] = in && s_valid[li] && (!in_pruned || insupp);
+                    const int64_t gb = s_gbase[li];
+                    v[u] = P.g[ok[u] ? gb + (int64_t)i * P.S + s0 + (rows ? 0 : c) : 0];
                 }
-            } else if (!(P.mode & FFT_IN_PRUNED) || insupp) {
-                v = P.g[s_gbase[li] + (int64_t)i * P.S + s0 + (rows ? 0 : c)];
+            }
+#pragma unroll
+            for (int u = 0; u < kFftLoadBatch; ++u) {
+                C2 w = ok[u] ? v[u] : cmake<T>(0, 0);
+                if constexpr (FROM_FHAT)
+                    w = cmake<T>((T)((double)w.x * fr[u]), (T)((double)w.y * fr[u]));
+                if (at[u] >= 0) buf[at[u]] = w;
             }
         }
-        buf[c * ldsL + i] = v;
-    }
+    };
+    if (P.mode & FFT_FROM_FHAT)
+        stage_in(std::true_type{});
+    else
+        stage_in(std::false_type{});
     __syncthreads();
 
     int logNs = 0;
     while (logNs + 2 <= logL) {
-        stage4<T>(buf, P.tw, logL, C, ldsL, logNs, P.sign, tid);
+        stage4<T, E>(buf, twl, logL, C, ldsL, logNs, P.sign, tid);
         logNs += 2;
     }
-    if (logNs < logL) stage2_last<T>(buf, P.tw, logL, C, ldsL, P.sign, tid);
+    if (logNs < logL) stage2_last<T, E>(buf, twl, logL, C, ldsL, P.sign, tid);
 
     for (int e = tid; e < total; e += kFftThreads) {
         int c, i;
@@ -258,36 +304,42 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
     }
 }
 
-template <typename T>
+template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
 {
-    fft_pass<T>(P, blockIdx.x);
+    fft_pass<T, E>(P, blockIdx.x);
 }
 
 // Passes of many transforms in one grid (a batch's jobs; LDS sized for the
 // largest).
-template <typename T>
+template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) void k_fft_pass_batch(const FftPass<T> *__restrict__ passes,
                                                                 const BlockMap *__restrict__ map)
 {
     const BlockMap bm = map[blockIdx.x];
-    fft_pass<T>(passes[bm.job], bm.blk);
+    fft_pass<T, E>(passes[bm.job], bm.blk);
 }
 
 template <typename T>
 void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipStream_t s)
 {
     if (blocks <= 0) return;
-    k_fft_pass<T><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
+    if ((p.C << p.logL) > kFftSmallElems)
+        k_fft_pass<T, fft_max_elems<T>()><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
+    else
+        k_fft_pass<T, kFftSmallElems><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
     CUNFFT_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_fft_batch(const FftPass<T> *passes, const BlockMap *map, uint32_t nblk,
-                      size_t lds_bytes, hipStream_t s)
+                      size_t lds_bytes, int max_elems, hipStream_t s)
 {
     if (nblk == 0) return;
-    k_fft_pass_batch<T><<<nblk, kFftThreads, lds_bytes, s>>>(passes, map);
+    if (max_elems > kFftSmallElems)
+        k_fft_pass_batch<T, fft_max_elems<T>()><<<nblk, kFftThreads, lds_bytes, s>>>(passes, map);
+    else
+        k_fft_pass_batch<T, kFftSmallElems><<<nblk, kFftThreads, lds_bytes, s>>>(passes, map);
     CUNFFT_HIP(hipGetLastError());
 }
 
@@ -350,22 +402,22 @@ FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *r
     } else {
         const int64_t cdef = sizeof(T) == 8 ? 8 : 16;
         C = std::min<int64_t>(cdef, P.S);
-        C = std::min<int64_t>(C, std::max<int64_t>(1, 4096 / P.L));
+        C = std::min<int64_t>(C, std::max<int64_t>(1, fft_max_elems<T>() / P.L));
         out.blocks = P.n_pa * (P.S / C);
         P.ldsL = C > 1 ? P.L + 1 : P.L;
     }
     P.C = (int)C;
     P.logC = ilog2_i(C);
-    out.lds = (size_t)C * P.ldsL * sizeof(cplx_t<T>);
+    out.lds = ((size_t)C * P.ldsL + P.L) * sizeof(cplx_t<T>);   // lines + twiddles
     return out;
 }
 
 template void launch_fft_pass<float>(const FftPass<float> &, int64_t, size_t, hipStream_t);
 template void launch_fft_pass<double>(const FftPass<double> &, int64_t, size_t, hipStream_t);
 template void launch_fft_batch<float>(const FftPass<float> *, const BlockMap *, uint32_t, size_t,
-                                      hipStream_t);
+                                      int, hipStream_t);
 template void launch_fft_batch<double>(const FftPass<double> *, const BlockMap *, uint32_t, size_t,
-                                       hipStream_t);
+                                       int, hipStream_t);
 template FftLaunch<float> fft_pass_setup<float>(const Geom &, const float2 *, const double *,
                                                 const int64_t *, int, bool, const float2 *,
                                                 float2 *, float2 *);

@@ -868,7 +868,11 @@ template <typename T, int D, int W, int V> struct Waves {
     static constexpr int tile_bytes = TileLayout<T, D, W, 1, V>::CELLS * TileLayout<T, D, W, 1, V>::CB;
     static constexpr bool alone = 2 * (tile_bytes + 1024) > 160 * 1024;
     static constexpr int gather = alone ? 8 : 4;
-    static constexpr int spread = alone ? 4 : 2;
+    // 2D: a node's window is W rows of W cells, which one wavefront covers
+    // with most lanes busy; splitting its rows over wavefronts would leave
+    // most lanes idle, so the adjoint runs one wavefront on a small tile
+    // (enough of them fit a CU's LDS to hide the latency)
+    static constexpr int spread = D == 2 && tile_bytes <= 16384 ? 1 : (alone ? 4 : 2);
 };
 
 template <typename T> WinConst<T> win_const(const Geom &ge)

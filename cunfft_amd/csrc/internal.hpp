@@ -89,10 +89,11 @@ struct BlockMap {
     uint32_t job;
     uint32_t blk;
 };
-// One launch of the passes of many transforms (one FftPass per job).
+// One launch of the passes of many transforms (one FftPass per job);
+// max_elems = the largest C * L of the passes.
 template <typename T>
 void launch_fft_batch(const FftPass<T> *passes, const BlockMap *map, uint32_t nblk,
-                      size_t lds_bytes, hipStream_t s);
+                      size_t lds_bytes, int max_elems, hipStream_t s);
 
 // ------------------------------------------------------------ gridding ---
 template <typename T>

@@ -475,7 +475,7 @@ static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
 // ------------------------------------------------------------ autotuner --
 
 // Times every configuration that applies on the plan's current nodes (best of
-// reps forward and adjoint passes over zero data, event-timed on s) and keeps
+// reps forward and adjoint passes over constant data, event-timed on s) and keeps
 // the fastest per tuned direction.  The binning of each candidate is built
 // before its timed passes: it is paid once per node set.  ms_out: forward
 // times of the configurations, then adjoint times (-1: not timed).
@@ -498,8 +498,12 @@ static void autotune(typename Traits<T>::plan *p, Impl<T> *im, int dirs, int rep
     try {
         CUNFFT_HIP(hipMalloc(&fh, sizeof(cplx_t<T>) * p->N_total));
         CUNFFT_HIP(hipMalloc(&f, sizeof(cplx_t<T>) * p->M_total));
-        CUNFFT_HIP(hipMemsetAsync(fh, 0, sizeof(cplx_t<T>) * p->N_total, s));
-        CUNFFT_HIP(hipMemsetAsync(f, 0, sizeof(cplx_t<T>) * p->M_total, s));
+        // nonzero data (words 0x3ff00000: 1.875f, or 1.0000002 as a double):
+        // the node-lane scatter skips its global atomics for zero partial sums
+        CUNFFT_HIP(hipMemsetD32Async((hipDeviceptr_t)fh, 0x3ff00000,
+                                     sizeof(cplx_t<T>) / 4 * p->N_total, s));
+        CUNFFT_HIP(hipMemsetD32Async((hipDeviceptr_t)f, 0x3ff00000,
+                                     sizeof(cplx_t<T>) / 4 * p->M_total, s));
         float best_ms[2] = {-1.0f, -1.0f};
         for (int c = 0; c < NC; ++c) {
             ms[0][c] = ms[1][c] = -1.0f;

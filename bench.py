@@ -389,7 +389,8 @@ def main():
     ach = grid_bytes(8) / (t32[kern] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": names[kern], "achieved": ach, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": measured_traffic(kern) if (cf, ca) == (0, 0) else None,
+            "traffic": measured_traffic(kern) if (cf if kern == "fwd_gather_ms" else ca) == 0
+            else None,
             "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
     cpu = None
     if not a.no_cpu_baseline:

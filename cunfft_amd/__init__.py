@@ -36,6 +36,7 @@ NO_HOST_BUFFERS = 1 << 9
 ATOMIC_SCATTER = 1 << 10
 PLAIN_GATHER = 1 << 11
 NODE_LANES = 1 << 12
+COMPENSATED = 1 << 13
 DEFAULT_CUT_OFF = 6
 TILE_SHAPES = 3                     # plane-tiled configurations 0 .. TILE_SHAPES-1
 TUNE_CONFIGS = TILE_SHAPES + 1      # + the node-lane configuration

@@ -214,8 +214,8 @@ bool Binning::plan(const Geom &ge, int elem_bytes_, int kernel_variant, int tile
     if (planes()) {
         // plane-tiled kernels: 32-bit grid offsets and the LDS tile must fit
         if (ncells > (int64_t(1) << 32)) return false;
-        const bool fits = elem_bytes == 4 ? tile_kernels_fit<float>(ge, tile_variant)
-                                          : tile_kernels_fit<double>(ge, tile_variant);
+        const bool fits = elem_bytes == 4 ? tile_kernels_fit<float>(ge, tile_variant, compensated)
+                                          : tile_kernels_fit<double>(ge, tile_variant, false);
         if (!fits) return false;
         slot_cells = elem_bytes == 4 ? tile_slot_cells<float>(ge, tile_variant)
                                      : tile_slot_cells<double>(ge, tile_variant);

@@ -87,6 +87,7 @@ struct Binning {
     void *fs = nullptr;             // M: adjoint coefficients in sorted order
     int elem_bytes = 8;             // bytes of one real component (4: FP32 plan, 8: FP64)
     bool own_scratch = true;        // allocate slots and fs (a batch brings its own)
+    bool compensated = false;       // FP32 Kahan tile sums and Neumaier fold (CUNFFT_COMPENSATED)
 
     // Decide tile shape for this geometry; false if the tiled path does not apply.
     bool plan(const Geom &ge, int elem_bytes, int kernel_variant, int tile_variant = 0);
@@ -166,7 +167,7 @@ bool launch_scatter_binned(const Geom &ge, const Binning &b, const double *x, co
                            cplx_t<T> *g, hipStream_t s, hipEvent_t mid = nullptr);
 
 // Plane-tiled kernels for d = 2, 3 (grid_tile.hip).
-template <typename T> bool tile_kernels_fit(const Geom &ge, int variant);
+template <typename T> bool tile_kernels_fit(const Geom &ge, int variant, bool comp);
 template <typename T> int64_t tile_slot_cells(const Geom &ge, int variant);
 template <typename T>
 void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cplx_t<T> *f,

@@ -333,6 +333,29 @@ __device__ __forceinline__ double rlane_t(double v, int k)
     return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
+// Kahan step: s + e keeps the running sum to about twice the working
+// precision (e holds the negated low part lost by the last additions).
+template <typename T> __device__ __forceinline__ void kahan_add(T &s, T &e, T x)
+{
+    const T y = x - e;
+    const T t = s + y;
+    e = (t - s) - y;
+    s = t;
+}
+
+// Fold sum: plain, or Neumaier's compensated sum (c collects the low parts;
+// the result is s + c).
+template <bool COMP, typename T> __device__ __forceinline__ void fold_add(T &s, T &c, T x)
+{
+    if constexpr (COMP) {
+        const T t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    } else {
+        s += x;
+    }
+}
+
 // ------------------------------------------------------------ gather ----
 // NW wavefronts share the staged tile and take batches of the chunk's nodes
 // round-robin.  Each lane accumulates its unit over the node's planes and
@@ -522,7 +545,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather_batch(const GridJob<T> 
 // read-add-write sequences stay in program order.  When a plane group can
 // reach past the window, the lanes concerned read and write a private dummy
 // unit past the tile.  fs holds the coefficients in sorted node order.
-template <typename T, int D, int W, int NW, int V>
+template <typename T, int D, int W, int NW, int V, bool COMP>
 __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc,
                                             const cplx_t<T> *__restrict__ fs, const Chunk ch,
                                             cplx_t<T> *__restrict__ slot, const WinConst<T> &wc)
@@ -535,8 +558,10 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
     constexpr int PB = L::PB;
     constexpr bool DUMMY = L::NEED_DUMMY;
     constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
-    constexpr int NB = batch_nodes<T, D, W>(TILE_UNITS * 16, NW, 0);
-    __shared__ __align__(16) VT tile[TILE_UNITS];
+    constexpr int NB = batch_nodes<T, D, W>(TILE_UNITS * 16 * (COMP ? 2 : 1), NW, 0);
+    // COMP: a second tile of Kahan compensation terms follows the sums
+    constexpr uint32_t EOFF = COMP ? TILE_UNITS * 16 : 0;
+    __shared__ __align__(16) VT tile[COMP ? 2 * TILE_UNITS : TILE_UNITS];
     __shared__ WinBatch<T, D, W, NB> wbs[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -550,7 +575,7 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
         VT z;
         if constexpr (CPL == 2) z = make_float4(0.f, 0.f, 0.f, 0.f);
         else z = make_double2(0.0, 0.0);
-        for (int idx = tid; idx < TILE_UNITS; idx += NT) tile[idx] = z;
+        for (int idx = tid; idx < (COMP ? 2 : 1) * TILE_UNITS; idx += NT) tile[idx] = z;
     }
     __syncthreads();
 
@@ -584,7 +609,7 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
                 const int cw = it[i].u2 - par + 1;
 #pragma unroll
                 for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
-                    VT v[PB];
+                    VT v[PB], ev[PB];
                     uint32_t ad[PB];
 #pragma unroll
                     for (int j = 0; j < PB; ++j) {
@@ -598,6 +623,7 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
                             }
                         }
                         v[j] = *reinterpret_cast<const VT *>(tb + ad[j]);
+                        if constexpr (COMP) ev[j] = *reinterpret_cast<const VT *>(tb + EOFF + ad[j]);
                     }
                     // the node's weights are read only after its tile reads are
                     // issued, and all of them before any use: one LDS round trip
@@ -625,7 +651,16 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
                         const int gq = g0 + j;
                         if (gq >= L::NGRP) break;
                         const T w0a = w0r[j];
-                        if constexpr (CPL == 2) {
+                        if constexpr (COMP) {
+                            // Kahan: e carries the low-order part lost by the sums
+                            kahan_add(v[j].x, ev[j].x, w0a * fux);
+                            kahan_add(v[j].y, ev[j].y, w0a * fuy);
+                            if constexpr (CPL == 2) {
+                                kahan_add(v[j].z, ev[j].z, w0a * fvx);
+                                kahan_add(v[j].w, ev[j].w, w0a * fvy);
+                            }
+                            *reinterpret_cast<VT *>(tb + EOFF + ad[j]) = ev[j];
+                        } else if constexpr (CPL == 2) {
                             v[j].x += w0a * fux;
                             v[j].y += w0a * fuy;
                             v[j].z += w0a * fvx;
@@ -663,8 +698,13 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
                         c = c < n ? c : n - 1;
                         const int r0 = c / (E1 * EL), rr = c - r0 * (E1 * EL);
                         const int r1 = rr / EL, rL = rr - r1 * EL;
-                        v[u] = tc[(q0 * L::T0 + r0) * L::PP + (q1 * L::T1 + r1) * L::RP +
-                                  qL * L::TL + rL];
+                        const int src = (q0 * L::T0 + r0) * L::PP + (q1 * L::T1 + r1) * L::RP +
+                                        qL * L::TL + rL;
+                        v[u] = tc[src];
+                        if constexpr (COMP) {
+                            const C e = tc[src + EOFF / sizeof(C)];
+                            v[u] = cmake<T>(v[u].x - e.x, v[u].y - e.y);
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
@@ -675,7 +715,7 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
             }
 }
 
-template <typename T, int D, int W, int NW, int V>
+template <typename T, int D, int W, int NW, int V, bool COMP>
 __global__ __launch_bounds__(NW * 64) void k_tile_spread(
     const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
     const Chunk *__restrict__ chunks, const uint32_t *__restrict__ n_chunks,
@@ -683,8 +723,8 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
 {
     using L = TileLayout<T, D, W, NW, V>;
     if (blockIdx.x >= n_chunks[0]) return;
-    tile_spread<T, D, W, NW, V>(desc, fs, chunks[blockIdx.x], slots + (int64_t)blockIdx.x * L::SC,
-                                wc);
+    tile_spread<T, D, W, NW, V, COMP>(desc, fs, chunks[blockIdx.x],
+                                      slots + (int64_t)blockIdx.x * L::SC, wc);
 }
 
 template <typename T, int D, int W, int NW, int V>
@@ -694,8 +734,8 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread_batch(const GridJob<T> 
     using L = TileLayout<T, D, W, NW, V>;
     const BlockMap bm = map[blockIdx.x];
     const GridJob<T> &J = jobs[bm.job];
-    tile_spread<T, D, W, NW, V>(static_cast<const NodeDesc<T> *>(J.desc), J.fs, J.chunks[bm.blk],
-                                J.slots + (int64_t)bm.blk * L::SC, J.wc);
+    tile_spread<T, D, W, NW, V, false>(static_cast<const NodeDesc<T> *>(J.desc), J.fs,
+                                       J.chunks[bm.blk], J.slots + (int64_t)bm.blk * L::SC, J.wc);
 }
 
 // Coefficients in sorted node order for the adjoint: fs[s] = f[perm[s]].
@@ -726,7 +766,7 @@ __global__ __launch_bounds__(256) void k_sorted_coeffs_batch(const GridJob<T> *_
 // candidate tiles of one leading-axis offset is loaded at once (a dense tile's
 // further chunks follow), and the sums run in a fixed order: the result does
 // not depend on scheduling.
-template <typename T, int D, int W, int PS, int V>
+template <typename T, int D, int W, int PS, int V, bool COMP>
 __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
                                           const uint32_t *__restrict__ tile_count,
                                           const uint32_t *__restrict__ chunk_start,
@@ -737,7 +777,8 @@ __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
     using C = cplx_t<T>;
     constexpr int CORE = L::T0 * L::T1 * L::TL;
     constexpr int CPT = CORE >= 512 ? 2 : (CORE + 255) / 256;   // cells per thread per slice
-    const int slice0 = slice * 256 * CPT;                          // first core cell of this slice
+    constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
+    const int slice0 = SLICES > 1 ? slice * 256 * CPT : 0;       // first core cell of this slice
     constexpr int QR = L::Q1 * L::QL;            // candidates per leading-axis offset
     uint32_t tk[3] = {0, 0, 0};
     {
@@ -755,7 +796,9 @@ __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
         r1[k] = (c / L::TL) % L::T1;
         rL[k] = c % L::TL;
     }
-    T sx[CPT], sy[CPT];
+    T sx[CPT], sy[CPT], cx[CPT], cy[CPT];   // sums and (COMP) Neumaier corrections
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) cx[k] = cy[k] = 0;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) sx[k] = sy[k] = 0;
 #pragma unroll
@@ -799,8 +842,8 @@ __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
 #pragma unroll
             for (int k = 0; k < CPT; ++k)
                 if (ok[qq][k]) {
-                    sx[k] += v[qq][k].x;
-                    sy[k] += v[qq][k].y;
+                    fold_add<COMP>(sx[k], cx[k], v[qq][k].x);
+                    fold_add<COMP>(sy[k], cy[k], v[qq][k].y);
                 }
         // further chunks of dense candidate tiles, four loads in flight
 #pragma unroll
@@ -819,8 +862,8 @@ __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
 #pragma unroll
                     for (int k = 0; k < CPT; ++k)
                         if (ok[qq][k] && c + u < nch[qq]) {
-                            sx[k] += w[u][k].x;
-                            sy[k] += w[u][k].y;
+                            fold_add<COMP>(sx[k], cx[k], w[u][k].x);
+                            fold_add<COMP>(sy[k], cy[k], w[u][k].y);
                         }
             }
     }
@@ -831,17 +874,18 @@ __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
             uint32_t cell = (tk[0] * L::T0 + r0[k]) * (D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : nL);
             if (D == 3) cell += (tk[1] * L::T1 + r1[k]) * nL;
             cell += tk[D - 1] * L::TL + rL[k];
-            g[cell] = cmake<T>(sx[k], sy[k]);
+            g[cell] = cmake<T>(sx[k] + cx[k], sy[k] + cy[k]);
         }
 }
 
-template <typename T, int D, int W, int PS, int V>
+template <typename T, int D, int W, int PS, int V, bool COMP>
 __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__ slots,
                                                    const uint32_t *__restrict__ tile_count,
                                                    const uint32_t *__restrict__ chunk_start,
                                                    cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
 {
-    tile_fold<T, D, W, PS, V>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x, blockIdx.y);
+    tile_fold<T, D, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x,
+                                    blockIdx.y);
 }
 
 // Batched fold: block blk of a job is tile blk / slices, slice blk % slices.
@@ -854,8 +898,8 @@ __global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__res
     constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
     const BlockMap bm = map[blockIdx.x];
     const GridJob<T> &J = jobs[bm.job];
-    tile_fold<T, D, W, PS, V>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg,
-                              bm.blk / SLICES, (int)(bm.blk % SLICES));
+    tile_fold<T, D, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg,
+                                     bm.blk / SLICES, (int)(bm.blk % SLICES));
 }
 
 // ------------------------------------------------------------ dispatch --
@@ -893,6 +937,14 @@ template <typename T, int D, int W, int V> constexpr bool fits_all()
            TileLayout<T, D, W, Waves<T, D, W, V>::spread, V>::fits;
 }
 
+// The compensated adjoint (FP32 only) holds a second tile of Kahan terms.
+template <typename T, int D, int W, int V> constexpr bool fits_comp()
+{
+    using LS = TileLayout<T, D, W, Waves<T, D, W, V>::spread, V>;
+    return sizeof(T) == 4 && fits_all<T, D, W, V>() &&
+           2L * (LS::CELLS * LS::CB + (LS::NEED_DUMMY ? 1024 : 0)) <= kLdsMax;
+}
+
 // Runs f.template run<D, W, V>() for the runtime window width, dimension and
 // tile variant; false when there is no such kernel.
 template <int D, int V, typename F> static bool with_w(int W, F &f)
@@ -927,21 +979,23 @@ template <typename F> static bool with_dwv(int d, int W, int v, F &f)
 }
 
 template <typename T> struct FitOp {
+    bool comp = false;
     int64_t slot_cells = 0;
     template <int D, int W, int V> bool run()
     {
         if constexpr (fits_all<T, D, W, V>()) {
             slot_cells = TileLayout<T, D, W, Waves<T, D, W, V>::spread, V>::SC;
-            return true;
+            return !comp || fits_comp<T, D, W, V>();
         } else {
             return false;
         }
     }
 };
 
-template <typename T> bool tile_kernels_fit(const Geom &ge, int variant)
+template <typename T> bool tile_kernels_fit(const Geom &ge, int variant, bool comp)
 {
     FitOp<T> op;
+    op.comp = comp;
     return with_dwv(ge.d, ge.W, variant, op);
 }
 
@@ -985,14 +1039,26 @@ template <typename T> struct SpreadOp {
             cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
             cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
             k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
-            k_tile_spread<T, D, W, NWS, V><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
-                static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
-                win_const<T>(ge));
-            if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
             using L = TileLayout<T, D, W, NWS, V>;
             constexpr int CORE = L::T0 * L::T1 * L::TL;
             constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
-            k_tile_fold<T, D, W, NWS, V><<<dim3((unsigned)b.ntiles, SLICES), 256, 0, s>>>(
+            const dim3 fold_grid((unsigned)b.ntiles, SLICES);
+            if constexpr (fits_comp<T, D, W, V>()) {
+                if (b.compensated) {
+                    k_tile_spread<T, D, W, NWS, V, true><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
+                        static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
+                        win_const<T>(ge));
+                    if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
+                    k_tile_fold<T, D, W, NWS, V, true><<<fold_grid, 256, 0, s>>>(
+                        slots, b.tile_count, b.chunk_start, g, ge, bin_geom(b));
+                    return true;
+                }
+            }
+            k_tile_spread<T, D, W, NWS, V, false><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
+                static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
+                win_const<T>(ge));
+            if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
+            k_tile_fold<T, D, W, NWS, V, false><<<fold_grid, 256, 0, s>>>(
                 slots, b.tile_count, b.chunk_start, g, ge, bin_geom(b));
             return true;
         } else {

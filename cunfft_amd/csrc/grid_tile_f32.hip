@@ -4,7 +4,7 @@
 
 namespace cunfft {
 
-template bool tile_kernels_fit<float>(const Geom &, int);
+template bool tile_kernels_fit<float>(const Geom &, int, bool);
 template int64_t tile_slot_cells<float>(const Geom &, int);
 template void launch_tile_gather<float>(const Geom &, const Binning &, const float2 *, float2 *,
                                        hipStream_t);

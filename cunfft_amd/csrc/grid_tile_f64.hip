@@ -4,7 +4,7 @@
 
 namespace cunfft {
 
-template bool tile_kernels_fit<double>(const Geom &, int);
+template bool tile_kernels_fit<double>(const Geom &, int, bool);
 template int64_t tile_slot_cells<double>(const Geom &, int);
 template void launch_tile_gather<double>(const Geom &, const Binning &, const double2 *, double2 *,
                                        hipStream_t);

@@ -111,6 +111,7 @@ template <typename T> struct Impl {
     Binning bins[2];
     bool dirty[2] = {true, true};
     int config[2] = {0, 0};  // gridding kernel configuration per direction
+    bool compensated = false;  // CUNFFT_COMPENSATED on an FP32 plan
     int slot(int dir) const { return config[1] == config[0] ? 0 : dir; }
     void nodes_changed() { dirty[0] = dirty[1] = true; }
 };
@@ -250,9 +251,10 @@ static void fg_exp_l(double *out, double b, int offset, int m)
 // with tile shape c, c == kTileVariants the node-lane kernels.
 static bool config_exists(int d, int c) { return c >= 0 && c <= kTileVariants && (d >= 2 || c == 0); }
 
-static bool plan_bins(Binning &b, const Geom &ge, int elem_bytes, int c)
+static bool plan_bins(Binning &b, const Geom &ge, int elem_bytes, int c, bool comp)
 {
     const bool lanes = c == kTileVariants;
+    b.compensated = comp && elem_bytes == 4;
     return b.plan(ge, elem_bytes, lanes ? 1 : 0, lanes ? 0 : c);
 }
 
@@ -266,8 +268,8 @@ template <typename T> static bool plan_configs(Impl<T> *im, int cf, int ca)
     im->nodes_changed();
     im->config[0] = cf;
     im->config[1] = ca;
-    bool ok = plan_bins(im->bins[0], im->geom, (int)sizeof(T), cf);
-    if (ca != cf) ok = plan_bins(im->bins[1], im->geom, (int)sizeof(T), ca) && ok;
+    bool ok = plan_bins(im->bins[0], im->geom, (int)sizeof(T), cf, im->compensated);
+    if (ca != cf) ok = plan_bins(im->bins[1], im->geom, (int)sizeof(T), ca, im->compensated) && ok;
     return ok;
 }
 
@@ -326,6 +328,7 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
     }
     im->geom = make_geom(d, NN, nn3, p->M_total, p->m);
     const int c0 = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
+    im->compensated = (p->flags & CUNFFT_COMPENSATED) && sizeof(T) == 4;
     plan_configs<T>(im, c0, c0);
     im->roc = make_rolloff(im->geom, im->rocoff);
     init_device_para<T>(p);
@@ -911,7 +914,7 @@ template <typename T> int api_set_config(typename Traits<T>::plan *p, int dirs, 
         return CUNFFT_ERR_INVALID_ARG;
     }
     Binning probe;
-    if (!plan_bins(probe, im->geom, (int)sizeof(T), c)) {
+    if (!plan_bins(probe, im->geom, (int)sizeof(T), c, im->compensated)) {
         report("the tiled kernels of this configuration do not fit the plan's window and grid");
         return CUNFFT_ERR_UNSUPPORTED;
     }

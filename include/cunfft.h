@@ -64,6 +64,12 @@ extern "C" {
 /** Tiled gridding with one node per lane instead of one lane per tap of the
  *  window row (d >= 2). */
 #define CUNFFT_NODE_LANES     (1U << 12)
+/** FP32 plans: compensated summation in the adjoint tiles -- Kahan sums in
+ *  the plane-tiled accumulation (a second LDS tile holds the compensation
+ *  terms) and Neumaier sums in the fold.  Costs LDS space and time; FP64
+ *  plans ignore it.  Configurations whose doubled tile does not fit the
+ *  LDS (FP32 3D 16x16x8) are not offered. */
+#define CUNFFT_COMPENSATED    (1U << 13)
 
 /* status codes */
 #define CUNFFT_SUCCESS          0

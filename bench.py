@@ -122,6 +122,63 @@ def time_plan(cu, precision, x, fh, fv, steps, warmup):
                "adj_fft_ms": med["adj_fft"]}
 
 
+def pcie_inclusive(cu, x, fh, fv, steps=4):
+    """FP32 steps through the host-buffer boundary (copy in, transform, copy out,
+    then the same for the adjoint), the reference's call sequence: synchronous,
+    and with CUNFFT_ASYNC_COPY on two plans whose copies and kernels overlap."""
+    def make(flags):
+        p = cu.Plan(N3, M_NODES, n=NN3, m=CUT, precision="single", flags=flags)
+        p.x[:] = x
+        p.f_hat[:] = fh
+        p.f[:] = fv
+        return p
+
+    def step(p):
+        p.copyDataToDevice()
+        p.transform()
+        p.copyDataToHost()
+        p.copyDataToDeviceAd()
+        p.adjoint()
+        p.copyDataToHostAd()
+
+    base = cu.CUFFT_INIT | cu.CUFFT_BOTH
+    p = make(base)
+    step(p)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(p)
+    ms_sync = 1e3 * (time.perf_counter() - t0) / steps
+    p.finalize()
+    pa, pb = make(base | cu.ASYNC_COPY), make(base | cu.ASYNC_COPY)
+
+    def two():
+        # forward of both plans queued before either result is read back
+        for q in (pa, pb):
+            q.copyDataToDevice()
+            q.transform()
+        for q in (pa, pb):
+            q.copyDataToHost()
+        for q in (pa, pb):
+            q.copyDataToDeviceAd()
+            q.adjoint()
+        for q in (pa, pb):
+            q.copyDataToHostAd()
+
+    two()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        two()
+    ms_async = 1e3 * (time.perf_counter() - t0) / (2 * steps)
+    pa.finalize()
+    pb.finalize()
+    return {"sync_ms_per_step": ms_sync, "sync_nodes_per_s": M_NODES / (ms_sync * 1e-3),
+            "async_two_plans_ms_per_step": ms_async,
+            "async_two_plans_nodes_per_s": M_NODES / (ms_async * 1e-3),
+            # x twice (d doubles per node), f twice and f_hat twice (FP32 complex)
+            "bytes_over_pcie_per_step": 2 * 24 * M_NODES + 2 * 8 * M_NODES
+            + 2 * 8 * int(np.prod(N3))}
+
+
 def accuracy(cu, p32, p64, x, fh, n_sample=4096):
     """Precision and window error on the bench inputs (GPU, untimed)."""
     import torch
@@ -368,6 +425,9 @@ def main():
     log(f"bench: {acc}")
     p32.finalize()
     p64.finalize()
+    log("bench: PCIe-inclusive steps")
+    pcie = pcie_inclusive(cu, x, fh, fv)
+    log(f"bench: {pcie}")
     batch = None
     if not a.no_batch:
         log("bench: batched 2D workload")
@@ -407,7 +467,7 @@ def main():
         "fp64": {"value": value64, "ms_per_step": t64["ms_per_step"],
                  "fp32_speedup_over_fp64": value / value64},
         "steps_fp32_ms": t32, "steps_fp64_ms": t64, "accuracy": acc,
-        "roofline": roof, "cpu_baseline": cpu, "batched_2d": batch,
+        "roofline": roof, "cpu_baseline": cpu, "batched_2d": batch, "pcie_inclusive": pcie,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -37,6 +37,7 @@ ATOMIC_SCATTER = 1 << 10
 PLAIN_GATHER = 1 << 11
 NODE_LANES = 1 << 12
 COMPENSATED = 1 << 13
+ASYNC_COPY = 1 << 14
 DEFAULT_CUT_OFF = 6
 TILE_SHAPES = 3                     # plane-tiled configurations 0 .. TILE_SHAPES-1
 TUNE_CONFIGS = TILE_SHAPES + 1      # + the node-lane configuration
@@ -149,6 +150,9 @@ def lib():
     for s in ("cundft_transform_rows_dev", "cundft_adjoint_cols_dev",
               "cunfftf_ndft_transform_rows_dev", "cunfftf_ndft_adjoint_cols_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, ctypes.c_longlong, vp, vp]
+        getattr(L, s).restype = ctypes.c_int
+    for s in ("cunfft_synchronize", "cunfftf_synchronize"):
+        getattr(L, s).argtypes = [PP]
         getattr(L, s).restype = ctypes.c_int
     for s in ("cunfft_autotune", "cunfftf_autotune"):
         getattr(L, s).argtypes = [PP, ctypes.c_int, ctypes.c_int, vp, vp]
@@ -395,6 +399,10 @@ class Plan:
         self._dev("cunfft_backward_dev", "cunfftf_backward_dev", ctypes.c_void_p(fhat_ptr),
                   ctypes.c_void_p(grad_f_ptr), ctypes.c_void_p(grad_fhat_ptr),
                   ctypes.c_void_p(grad_x_ptr), ctypes.c_void_p(stream))
+
+    def synchronize(self):
+        """Wait for the plan's queued work (ASYNC_COPY plans; cunfft_synchronize)."""
+        self._dev("cunfft_synchronize", "cunfftf_synchronize")
 
     # ------------------------------------------------------------ autotuner
     def autotune(self, directions=TUNE_FORWARD | TUNE_ADJOINT, reps=5, stream=None):

@@ -551,14 +551,16 @@ static void autotune(typename Traits<T>::plan *p, Impl<T> *im, int dirs, int rep
             for (int c = 0; c < NC; ++c) ms_out[dir * NC + c] = ms[dir][c];
 }
 
+// A copy on the plan stream; it returns after the copy unless `async` (and
+// not timing).
 template <typename T>
 static double timed_copy(Impl<T> *im, bool timing, void *dst, const void *src, size_t bytes,
-                         hipMemcpyKind kind)
+                         hipMemcpyKind kind, bool async = false)
 {
     if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], im->stream));
     CUNFFT_HIP(hipMemcpyAsync(dst, src, bytes, kind, im->stream));
     if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], im->stream));
-    CUNFFT_HIP(hipStreamSynchronize(im->stream));
+    if (!async || timing) CUNFFT_HIP(hipStreamSynchronize(im->stream));
     if (!timing) return 0.0;
     float ms = 0;
     CUNFFT_HIP(hipEventElapsedTime(&ms, im->ev[0], im->ev[1]));
@@ -646,14 +648,16 @@ template <typename T> void api_copy_in(typename Traits<T>::plan *p, bool adjoint
         need_host<T>(p);
         need_device<T>(p);
         const bool timing = p->flags & CUNFFT_MEASURE_TIME;
+        const bool async = p->flags & CUNFFT_ASYNC_COPY;
         double t = 0;
         if (adjoint)
-            t += timed_copy(im, timing, p->f_gpu, p->f, sizeof(C2) * p->M_total, hipMemcpyHostToDevice);
+            t += timed_copy(im, timing, p->f_gpu, p->f, sizeof(C2) * p->M_total, hipMemcpyHostToDevice,
+                            async);
         else
             t += timed_copy(im, timing, p->fhat_gpu, p->f_hat, sizeof(C2) * p->N_total,
-                            hipMemcpyHostToDevice);
+                            hipMemcpyHostToDevice, async);
         t += timed_copy(im, timing, p->x_gpu, p->x, sizeof(double) * p->M_total * p->d,
-                        hipMemcpyHostToDevice);
+                        hipMemcpyHostToDevice, async);
         im->nodes_changed();
         p->CUNFFTTimes.time_COPY_IN = t;
     });
@@ -697,7 +701,7 @@ template <typename T> void api_transform(typename Traits<T>::plan *p)
         double tf = 0, tc = 0;
         run_forward<T>(p, im, reinterpret_cast<const cplx_t<T> *>(p->fhat_gpu),
                        reinterpret_cast<cplx_t<T> *>(p->f_gpu), im->stream, timing, &tf, &tc);
-        CUNFFT_HIP(hipStreamSynchronize(im->stream));
+        if (!(p->flags & CUNFFT_ASYNC_COPY)) CUNFFT_HIP(hipStreamSynchronize(im->stream));
         p->CUNFFTTimes.time_ROC = 0.0;
         p->CUNFFTTimes.time_FFT = tf;
         p->CUNFFTTimes.time_CONV = tc;
@@ -715,7 +719,7 @@ template <typename T> void api_adjoint(typename Traits<T>::plan *p)
         run_adjoint<T>(p, im, reinterpret_cast<const cplx_t<T> *>(p->f_gpu),
                        reinterpret_cast<cplx_t<T> *>(p->fhat_gpu), im->stream, timing, &tf, &tc,
                        &tfold);
-        CUNFFT_HIP(hipStreamSynchronize(im->stream));
+        if (!(p->flags & CUNFFT_ASYNC_COPY)) CUNFFT_HIP(hipStreamSynchronize(im->stream));
         p->CUNFFTTimes.time_ROC = 0.0;
         p->CUNFFTTimes.time_FFT = tf;
         p->CUNFFTTimes.time_CONV = tc;
@@ -924,6 +928,11 @@ template <typename T> int api_set_config(typename Traits<T>::plan *p, int dirs, 
     });
 }
 
+template <typename T> int api_synchronize(typename Traits<T>::plan *p)
+{
+    return guard(p, [&] { CUNFFT_HIP(hipStreamSynchronize(impl_of<T>(p)->stream)); });
+}
+
 template <typename T> void api_init_plan(int dim, uint_t *n, cunfftHandle *h)
 {
     if (!h) return;
@@ -1096,6 +1105,7 @@ int cunfft_autotune(cunfft_plan *p, int directions, int reps, float *ms, void *s
 {
     return api_autotune<double>(p, directions, reps, ms, s);
 }
+int cunfft_synchronize(cunfft_plan *p) { return api_synchronize<double>(p); }
 int cunfft_get_config(const cunfft_plan *p, int direction)
 {
     return api_get_config<double>(p, direction);
@@ -1213,6 +1223,7 @@ int cunfftf_autotune(cunfftf_plan *p, int directions, int reps, float *ms, void 
 {
     return api_autotune<float>(p, directions, reps, ms, s);
 }
+int cunfftf_synchronize(cunfftf_plan *p) { return api_synchronize<float>(p); }
 int cunfftf_get_config(const cunfftf_plan *p, int direction)
 {
     return api_get_config<float>(p, direction);

@@ -70,6 +70,14 @@ extern "C" {
  *  plans ignore it.  Configurations whose doubled tile does not fit the
  *  LDS (FP32 3D 16x16x8) are not offered. */
 #define CUNFFT_COMPENSATED    (1U << 13)
+/** Asynchronous copy boundary: copyDataToDevice[Ad], cunfft_transform and
+ *  cunfft_adjoint return once their work is queued on the plan's stream
+ *  (the reference returns after it, src/cunfft.cu:107-224); copyDataToHost[Ad]
+ *  and the copy_*_ToHost calls still return with the data.  Several plans
+ *  then overlap one plan's PCIe copies with another's kernels.  The caller
+ *  must not change plan->x / f_hat / f until the plan's next copy to the
+ *  host or cunfft_synchronize.  CUNFFT_MEASURE_TIME synchronizes anyway. */
+#define CUNFFT_ASYNC_COPY     (1U << 14)
 
 /* status codes */
 #define CUNFFT_SUCCESS          0
@@ -250,6 +258,8 @@ int cunfft_grad_x_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
 int cunfft_backward_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
                         const cunfftDoubleComplex *grad_f_dev, cunfftDoubleComplex *grad_fhat_dev,
                         double *grad_x_dev, void *stream);
+/* Wait for all work queued on the plan's stream (CUNFFT_ASYNC_COPY plans). */
+int cunfft_synchronize(cunfft_plan *plan);
 /* Gridding kernel configurations (no counterpart in the reference, whose
  * tile shape and kernels are fixed at compile time, src/cunfft_kernel.cu).
  * Configuration c < CUNFFT_TILE_SHAPES is the plane-tiled kernel pair (one
@@ -370,6 +380,7 @@ int cunfftf_grad_x_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
 int cunfftf_backward_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                          const cunfftFloatComplex *grad_f_dev, cunfftFloatComplex *grad_fhat_dev,
                          float *grad_x_dev, void *stream);
+int cunfftf_synchronize(cunfftf_plan *plan);
 int cunfftf_autotune(cunfftf_plan *plan, int directions, int reps, float *ms_out, void *stream);
 int cunfftf_get_config(const cunfftf_plan *plan, int direction);
 int cunfftf_set_config(cunfftf_plan *plan, int directions, int config);
@@ -415,6 +426,7 @@ typedef float dTyp;
 #define cunfft_adjoint_dev cunfftf_adjoint_dev
 #define cunfft_grad_x_dev cunfftf_grad_x_dev
 #define cunfft_backward_dev cunfftf_backward_dev
+#define cunfft_synchronize cunfftf_synchronize
 #define cunfft_autotune cunfftf_autotune
 #define cunfft_get_config cunfftf_get_config
 #define cunfft_set_config cunfftf_set_config

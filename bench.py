@@ -9,7 +9,7 @@ The headline runs in FP32 complex with the FP64 index/fraction split and FP64
 roll-off factors; the same steps are timed in full FP64 beside it.
 
 Beside it (the "batched_2d" object): 96 uneven 2D FP32 problems, N_b x N_b
-with N_b in {64, 96, 128, 192, 256}, n_b = 2 next_pow2(N_b), m = 4, M_b
+with N_b in {64, 96, 128, 192, 256}, sigma = 2 (n_b = 2 N_b), m = 4, M_b
 uniform in [4096, 65536] nodes, 8 right-hand sides each; one step = forward +
 adjoint of every problem and right-hand side, as one graph launch of the
 batched kernels, against a loop of single-plan launches over the same data.
@@ -291,7 +291,7 @@ def bench_batch(cu, steps, warmup, seed=5150):
     N = np.stack([sizes, sizes], axis=1)
     M = r.integers(4096, 65536, P, endpoint=True)
     dev = torch.device("cuda")
-    b = Batch(N, M, m=m, n_rhs=R, precision="single")
+    b = Batch(N, M, m=m, n_rhs=R, n=2 * N, precision="single")   # sigma = 2 exactly
     xs = []
     for i in range(P):
         x = torch.from_numpy(r.uniform(-0.5, 0.5, (int(M[i]), 2))).to(dev)
@@ -365,8 +365,8 @@ def bench_batch(cu, steps, warmup, seed=5150):
     b.close()
     nodes = int(M.sum()) * R
     return {
-        "workload": "96 uneven 2D problems N_b^2, N_b in {64,96,128,192,256}, n_b = "
-                    "2 next_pow2(N_b), m=4, M_b uniform in [4096,65536], 8 right-hand sides, "
+        "workload": "96 uneven 2D problems N_b^2, N_b in {64,96,128,192,256}, sigma=2 "
+                    "(n_b = 2 N_b), m=4, M_b uniform in [4096,65536], 8 right-hand sides, "
                     "forward+adjoint, FP32",
         "value": nodes / (ms_batch * 1e-3), "unit": "nodes/s", "ms_per_step": ms_batch,
         "node_transforms_per_step": nodes, "loop_of_single_plans_ms_per_step": ms_loop,

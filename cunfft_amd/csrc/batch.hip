@@ -148,8 +148,9 @@ template <typename T> struct BatchImpl {
                 nn[t] = n ? n[(size_t)i * d + t] : 2 * v;   // src/cunfft.cu:296-299
                 if (NN[t] < 2 || (NN[t] & 1))
                     throw Error(CUNFFT_ERR_INVALID_ARG, "N_t must be even and >= 2");
-                if (nn[t] < NN[t] || (nn[t] & (nn[t] - 1)) || nn[t] > CUNFFT_MAX_AXIS)
-                    throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t must be a power of two in [N_t, 4096]");
+                if (nn[t] < NN[t] || !fft_length_ok(nn[t]))
+                    throw Error(CUNFFT_ERR_UNSUPPORTED,
+                                "n_t must be 2^a 3^b (a >= 1) in [N_t, 4096]");
                 if (nn[t] < W) throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= 2m+2");
             }
             if (M[i] < 1) throw Error(CUNFFT_ERR_INVALID_ARG, "M must be >= 1");
@@ -321,7 +322,7 @@ template <typename T> struct BatchImpl {
                     ph.push_back(f.P);
                     for (int64_t k = 0; k < f.blocks; ++k) mh.push_back(BlockMap{job, (uint32_t)k});
                     L.lds = std::max(L.lds, f.lds);
-                    L.max_elems = std::max(L.max_elems, f.P.C << f.P.logL);
+                    L.max_elems = std::max(L.max_elems, f.P.C * f.P.L);
                 }
             }
             L.nblk = (uint32_t)(mh.size() - L.map_off);

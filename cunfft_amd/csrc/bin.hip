@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
         int64_t u;
         double c;
         window_start(x[j * bg.d + t], ge.n[t], ge.m, u, c);
-        const uint32_t tc = (uint32_t)((u & (ge.n[t] - 1)) >> bg.logT[t]);
+        const uint32_t tc = (uint32_t)(wrap_lo(u, ge.n[t]) >> bg.logT[t]);
         key = key * (uint32_t)bg.nt[t] + tc;
     }
     keys[j] = key;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_bin_desc(const double *__restrict__ x,
         int64_t u;
         double c;
         window_start(x[(int64_t)j * bg.d + t], ge.n[t], ge.m, u, c);
-        const uint32_t lo = (uint32_t)((u & (ge.n[t] - 1)) - ((int64_t)tc[t] << bg.logT[t]));
+        const uint32_t lo = (uint32_t)(wrap_lo(u, ge.n[t]) - ((int64_t)tc[t] << bg.logT[t]));
         loc |= lo << (8 * t);
         nd.f[t] = (T)(c - (double)ge.m);
     }
@@ -204,7 +204,7 @@ bool Binning::plan(const Geom &ge, int elem_bytes_, int kernel_variant, int tile
     int64_t ncells = 1;
     for (int t = 0; t < 3; ++t) {
         T[t] = t < d ? shape[t] : 1;
-        if (t < d && ge.n[t] < T[t]) return false;
+        if (t < d && (ge.n[t] < T[t] || ge.n[t] % T[t] != 0)) return false;
         nt[t] = t < d ? ge.n[t] / T[t] : 1;
         ntiles *= nt[t];
         ncells *= t < d ? ge.n[t] : 1;
@@ -326,11 +326,11 @@ __device__ __forceinline__ int64_t tile_cell(int idx, const int64_t (&o)[3], con
     const int r = idx - p0 * (PD::P1 * PD::P2);
     const int p1 = r / PD::P2;
     const int p2 = r - p1 * PD::P2;
-    const int64_t c0 = (o[0] + p0) & (ge.n[0] - 1);
+    const int64_t c0 = wrap_hi(o[0] + p0, ge.n[0]);
     if constexpr (D == 1) return c0;
-    const int64_t c1 = (o[1] + p1) & (ge.n[1] - 1);
+    const int64_t c1 = wrap_hi(o[1] + p1, ge.n[1]);
     if constexpr (D == 2) return c0 * ge.n[1] + c1;
-    const int64_t c2 = (o[2] + p2) & (ge.n[2] - 1);
+    const int64_t c2 = wrap_hi(o[2] + p2, ge.n[2]);
     return (c0 * ge.n[1] + c1) * ge.n[2] + c2;
 }
 
@@ -347,7 +347,7 @@ __device__ __forceinline__ int node_local(const double *__restrict__ xs, uint32_
         int64_t u;
         double c;
         window_start(xs[(int64_t)s * D + t], ge.n[t], ge.m, u, c);
-        int lo = (int)((u & (ge.n[t] - 1)) - o[t]);
+        int lo = (int)(wrap_lo(u, ge.n[t]) - o[t]);
         lo = lo < 0 ? 0 : (lo >= TileShape<D>::T[t] ? TileShape<D>::T[t] - 1 : lo);
         window_weights<T, W>(c, ge.invb[t], ge.nrm[t], w[t]);
         base += lo * stride[t];

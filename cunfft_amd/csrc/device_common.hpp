@@ -43,6 +43,14 @@ __device__ __forceinline__ void window_start(double x, int64_t n, int m, int64_t
     c = nx - (double)u;
 }
 
+// Index wrap onto [0, n) for any grid length n (the reference masks with
+// n - 1, src/cunfft_kernel.cu:528-531, which needs a power of two).
+// wrap_lo: a in [-n, n); wrap_hi: a in [0, 2n).  Window starts lie in
+// [-n/2 - m - 1, n/2 - m + 1] and n >= 2m + 2, so one step suffices.
+__device__ __forceinline__ int64_t wrap_lo(int64_t a, int64_t n) { return a < 0 ? a + n : a; }
+__device__ __forceinline__ int64_t wrap_hi(int64_t a, int64_t n) { return a >= n ? a - n : a; }
+__device__ __forceinline__ uint32_t wrap_hi32(uint32_t a, uint32_t n) { return a >= n ? a - n : a; }
+
 template <typename T> __device__ __forceinline__ T win_exp(T v);
 template <> __device__ __forceinline__ float win_exp<float>(float v) { return expf(v); }
 template <> __device__ __forceinline__ double win_exp<double>(double v) { return exp(v); }

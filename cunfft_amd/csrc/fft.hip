@@ -7,8 +7,8 @@
 //            (src/cunfft_kernel.cu:264-467, src/cunfft_wrapper.cu:77-121)
 // A d-dimensional transform is d passes, one per axis.  Each workgroup loads
 // C whole lines of the axis into LDS, runs radix-4 (and one radix-2) Stockham
-// stages there, and writes the lines back: one HBM read and one write per
-// pass.  The forward passes skip lines that are still all zero and read only
+// stages there -- then radix-3 stages when L = 2^a 3^b is not a power of two
+// -- and writes the lines back: one HBM read and one write per pass.  The forward passes skip lines that are still all zero and read only
 // the N_t nonzero entries of each line; the first forward pass reads fhat and
 // applies exp(fac k^2) on the way in.  The adjoint passes write only the N_t
 // entries that survive the crop; the last one writes fhat with exp(fac k^2).
@@ -173,6 +173,88 @@ __device__ __forceinline__ void stage2_last(cplx_t<T> *buf, const cplx_t<T> *__r
     __syncthreads();
 }
 
+// a = q d + r for 0 <= a < 2^22 with rd = 1/d in FP32 (exact after one
+// correction step)
+__device__ __forceinline__ void divmod_small(int a, int d, float rd, int &q, int &r)
+{
+    q = (int)((float)a * rd);
+    r = a - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+}
+
+// R-point DFT with kernel exp(sign 2 pi i r q / R), R = 2, 3, 4.
+template <typename T, int R> __device__ __forceinline__ void dft_r(cplx_t<T> (&v)[R], int sign)
+{
+    if constexpr (R == 2) {
+        const cplx_t<T> a = v[0], b = v[1];
+        v[0] = cmake<T>(a.x + b.x, a.y + b.y);
+        v[1] = cmake<T>(a.x - b.x, a.y - b.y);
+    } else if constexpr (R == 3) {
+        // y1,2 = a - (b + c)/2 +- sign i (sqrt(3)/2)(b - c)
+        const T h = (T)0.86602540378443864676;   // sqrt(3) / 2
+        const cplx_t<T> a = v[0], b = v[1], c = v[2];
+        const T sx = b.x + c.x, sy = b.y + c.y;
+        const T dx = (T)sign * h * (b.x - c.x), dy = (T)sign * h * (b.y - c.y);
+        const T mx = a.x - (T)0.5 * sx, my = a.y - (T)0.5 * sy;
+        v[0] = cmake<T>(a.x + sx, a.y + sy);
+        v[1] = cmake<T>(mx - dy, my + dx);
+        v[2] = cmake<T>(mx + dy, my - dx);
+    } else {
+        dft4<T>(v, sign);
+    }
+}
+
+// One Stockham stage of radix R for any L (stride Q = L/R, Ns = product of
+// the radices before it): butterfly j of line c reads j + rQ, multiplies
+// input r by exp(sign 2 pi i r (j mod Ns) / (R Ns)), and writes its R-point
+// DFT to (j div Ns) R Ns + (j mod Ns) + r Ns.
+template <typename T, int E, int R>
+__device__ __forceinline__ void stage_any(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int L,
+                                          int C, int ldsL, int Ns, int sign, int tid)
+{
+    const int Q = L / R;
+    const int nb = C * Q;
+    const int tws = L / (R * Ns);    // twiddle-table step of exp(2 pi i / (R Ns))
+    const float rQ = 1.0f / (float)Q, rN = 1.0f / (float)Ns;
+    constexpr int KB = (E / R + kFftThreads - 1) / kFftThreads;
+    cplx_t<T> v[KB][R];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            int c, j, jq, jr;
+            divmod_small(b, Q, rQ, c, j);
+            divmod_small(j, Ns, rN, jq, jr);
+            const cplx_t<T> *base = buf + c * ldsL + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[k][r] = base[r * Q];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[k][r] = cmul<T>(v[k][r], twiddle<T>(tw, r * jr * tws, sign));
+            dft_r<T, R>(v[k], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            int c, j, jq, jr;
+            divmod_small(b, Q, rQ, c, j);
+            divmod_small(j, Ns, rN, jq, jr);
+            cplx_t<T> *ob = buf + c * ldsL + jq * (R * Ns) + jr;
+#pragma unroll
+            for (int r = 0; r < R; ++r) ob[r * Ns] = v[k][r];
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T, int E>
 __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
 {
@@ -208,7 +290,9 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
     for (int q = tid; q < L; q += kFftThreads) twl[q] = P.tw[q];
     __syncthreads();
 
-    const int total = C << logL;
+    const bool pow2 = logL >= 0;
+    const float rL = 1.0f / (float)L;
+    const int total = C * L;
     const int Nt = (int)(P.t == 0 ? P.N[0] : (P.t == 1 ? P.N[1] : P.N[2]));
     const int hN = Nt >> 1;
     const double *roct =
@@ -232,8 +316,12 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
                 const int e = e0 + u * kFftThreads;
                 int c, i;
                 if (rows) {
-                    c = e >> logL;
-                    i = e & (L - 1);
+                    if (pow2) {
+                        c = e >> logL;
+                        i = e & (L - 1);
+                    } else {
+                        divmod_small(e, L, rL, c, i);
+                    }
                 } else {
                     c = e & (C - 1);
                     i = e >> logC;
@@ -271,18 +359,42 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
         stage_in(std::false_type{});
     __syncthreads();
 
-    int logNs = 0;
-    while (logNs + 2 <= logL) {
-        stage4<T, E>(buf, twl, logL, C, ldsL, logNs, P.sign, tid);
-        logNs += 2;
+    if (pow2) {
+        int logNs = 0;
+        while (logNs + 2 <= logL) {
+            stage4<T, E>(buf, twl, logL, C, ldsL, logNs, P.sign, tid);
+            logNs += 2;
+        }
+        if (logNs < logL) stage2_last<T, E>(buf, twl, logL, C, ldsL, P.sign, tid);
+    } else {
+        // L = 2^a 3^b: radix 4 while the power-of-two part allows, one radix 2
+        // if a is odd, then b radix-3 stages
+        int p2 = L;
+        for (int k = 0; k < P.n3; ++k) p2 /= 3;
+        int Ns = 1;
+        while (Ns * 4 <= p2) {
+            stage_any<T, E, 4>(buf, twl, L, C, ldsL, Ns, P.sign, tid);
+            Ns *= 4;
+        }
+        if (Ns < p2) {
+            stage_any<T, E, 2>(buf, twl, L, C, ldsL, Ns, P.sign, tid);
+            Ns *= 2;
+        }
+        for (int k = 0; k < P.n3; ++k) {
+            stage_any<T, E, 3>(buf, twl, L, C, ldsL, Ns, P.sign, tid);
+            Ns *= 3;
+        }
     }
-    if (logNs < logL) stage2_last<T, E>(buf, twl, logL, C, ldsL, P.sign, tid);
 
     for (int e = tid; e < total; e += kFftThreads) {
         int c, i;
         if (rows) {
-            c = e >> logL;
-            i = e & (L - 1);
+            if (pow2) {
+                c = e >> logL;
+                i = e & (L - 1);
+            } else {
+                divmod_small(e, L, rL, c, i);
+            }
         } else {
             c = e & (C - 1);
             i = e >> logC;
@@ -324,7 +436,7 @@ template <typename T>
 void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipStream_t s)
 {
     if (blocks <= 0) return;
-    if ((p.C << p.logL) > kFftSmallElems)
+    if (p.C * p.L > kFftSmallElems)
         k_fft_pass<T, fft_max_elems<T>()><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
     else
         k_fft_pass<T, kFftSmallElems><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
@@ -348,6 +460,13 @@ static int ilog2_i(int64_t v)
     int l = 0;
     while ((int64_t(1) << l) < v) ++l;
     return l;
+}
+
+static int64_t prev_pow2_i(int64_t v)
+{
+    int64_t p = 1;
+    while (2 * p <= v) p <<= 1;
+    return p;
 }
 
 static int64_t next_pow2_i(int64_t v)
@@ -381,7 +500,9 @@ FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *r
     P.d = d;
     P.t = t;
     P.L = (int)ge.n[t];
-    P.logL = ilog2_i(P.L);
+    P.logL = (P.L & (P.L - 1)) == 0 ? ilog2_i(P.L) : -1;
+    P.n3 = 0;
+    for (int l = P.L; l % 3 == 0; l /= 3) ++P.n3;
     P.S = 1;
     for (int q = t + 1; q < d; ++q) P.S *= ge.n[q];
     P.n_pa = 1;
@@ -391,18 +512,20 @@ FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *r
         P.mode = (t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED;
     else
         P.mode = (t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED;
+    // C (a power of two) lines per workgroup
     const int64_t target = 2048;
     int64_t C;
     if (P.S == 1) {
-        C = P.L >= target ? 1 : target / P.L;
+        C = P.L >= target ? 1 : prev_pow2_i(target / P.L);
         C = std::min<int64_t>(C, 64);
         C = std::min<int64_t>(C, next_pow2_i(P.n_pa));
         out.blocks = (P.n_pa + C - 1) / C;
         P.ldsL = P.L;
     } else {
+        // adjacent columns: C divides S (the largest power of two in S bounds it)
         const int64_t cdef = sizeof(T) == 8 ? 8 : 16;
-        C = std::min<int64_t>(cdef, P.S);
-        C = std::min<int64_t>(C, std::max<int64_t>(1, fft_max_elems<T>() / P.L));
+        C = std::min<int64_t>(cdef, P.S & -P.S);
+        C = std::min<int64_t>(C, std::max<int64_t>(1, prev_pow2_i(fft_max_elems<T>() / P.L)));
         out.blocks = P.n_pa * (P.S / C);
         P.ldsL = C > 1 ? P.L + 1 : P.L;
     }

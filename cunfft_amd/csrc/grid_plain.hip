@@ -36,6 +36,7 @@ __device__ __forceinline__ void load_window(const Geom &ge, const double *__rest
     for (int t = 0; t < D; ++t) {
         double c;
         window_start(x[j * D + t], ge.n[t], ge.m, nw.u[t], c);
+        nw.u[t] = wrap_lo(nw.u[t], ge.n[t]);
         if constexpr (WS <= 18) {
             window_weights<T, WS>(c, ge.invb[t], ge.nrm[t], nw.w[t]);
         } else {
@@ -59,27 +60,27 @@ __global__ __launch_bounds__(kBlock) void k_gather_plain(const cplx_t<T> *__rest
     if (j >= ge.M) return;
     NodeWin<T, D, WS> nw;
     load_window<T, D, WS>(ge, x, j, W, nw);
-    const int64_t m0 = ge.n[0] - 1;
+    const int64_t n0 = ge.n[0];
     T sr = 0, si = 0;
     if constexpr (D == 1) {
 #pragma unroll
         for (int a = 0; a < WS; ++a) {
             if (WT == 0 && a >= W) break;
-            const cplx_t<T> v = g[(nw.u[0] + a) & m0];
+            const cplx_t<T> v = g[wrap_hi(nw.u[0] + a, n0)];
             sr += v.x * nw.w[0][a];
             si += v.y * nw.w[0][a];
         }
     } else if constexpr (D == 2) {
-        const int64_t n1 = ge.n[1], m1 = n1 - 1;
+        const int64_t n1 = ge.n[1];
 #pragma unroll
         for (int a = 0; a < WS; ++a) {
             if (WT == 0 && a >= W) break;
-            const cplx_t<T> *row = g + ((nw.u[0] + a) & m0) * n1;
+            const cplx_t<T> *row = g + (wrap_hi(nw.u[0] + a, n0)) * n1;
             T r1 = 0, i1 = 0;
 #pragma unroll
             for (int c = 0; c < WS; ++c) {
                 if (WT == 0 && c >= W) break;
-                const cplx_t<T> v = row[(nw.u[1] + c) & m1];
+                const cplx_t<T> v = row[wrap_hi(nw.u[1] + c, n1)];
                 r1 += v.x * nw.w[1][c];
                 i1 += v.y * nw.w[1][c];
             }
@@ -87,20 +88,20 @@ __global__ __launch_bounds__(kBlock) void k_gather_plain(const cplx_t<T> *__rest
             si += i1 * nw.w[0][a];
         }
     } else {
-        const int64_t n1 = ge.n[1], m1 = n1 - 1, n2 = ge.n[2], m2 = n2 - 1;
+        const int64_t n1 = ge.n[1], n2 = ge.n[2];
 #pragma unroll 1
         for (int a = 0; a < W; ++a) {
-            const cplx_t<T> *plane = g + ((nw.u[0] + a) & m0) * n1 * n2;
+            const cplx_t<T> *plane = g + (wrap_hi(nw.u[0] + a, n0)) * n1 * n2;
             T r1 = 0, i1 = 0;
 #pragma unroll
             for (int c = 0; c < WS; ++c) {
                 if (WT == 0 && c >= W) break;
-                const cplx_t<T> *row = plane + ((nw.u[1] + c) & m1) * n2;
+                const cplx_t<T> *row = plane + (wrap_hi(nw.u[1] + c, n1)) * n2;
                 T r2 = 0, i2 = 0;
 #pragma unroll
                 for (int e = 0; e < WS; ++e) {
                     if (WT == 0 && e >= W) break;
-                    const cplx_t<T> v = row[(nw.u[2] + e) & m2];
+                    const cplx_t<T> v = row[wrap_hi(nw.u[2] + e, n2)];
                     r2 += v.x * nw.w[2][e];
                     i2 += v.y * nw.w[2][e];
                 }
@@ -143,45 +144,45 @@ __global__ __launch_bounds__(kBlock) void k_scatter_atomic(const double *__restr
     NodeWin<T, D, WS> nw;
     load_window<T, D, WS>(ge, x, j, W, nw);
     const cplx_t<T> fj = f[j];
-    const int64_t m0 = ge.n[0] - 1;
+    const int64_t n0 = ge.n[0];
     if constexpr (D == 1) {
 #pragma unroll
         for (int a = 0; a < WS; ++a) {
             if (WT == 0 && a >= W) break;
-            atomic_add_c(g + ((nw.u[0] + a) & m0), fj.x * nw.w[0][a], fj.y * nw.w[0][a]);
+            atomic_add_c(g + (wrap_hi(nw.u[0] + a, n0)), fj.x * nw.w[0][a], fj.y * nw.w[0][a]);
         }
     } else if constexpr (D == 2) {
-        const int64_t n1 = ge.n[1], m1 = n1 - 1;
+        const int64_t n1 = ge.n[1];
 #pragma unroll
         for (int a = 0; a < WS; ++a) {
             if (WT == 0 && a >= W) break;
-            cplx_t<T> *row = g + ((nw.u[0] + a) & m0) * n1;
+            cplx_t<T> *row = g + (wrap_hi(nw.u[0] + a, n0)) * n1;
             const T fr = fj.x * nw.w[0][a], fi = fj.y * nw.w[0][a];
 #pragma unroll
             for (int c = 0; c < WS; ++c) {
                 if (WT == 0 && c >= W) break;
-                atomic_add_c(row + ((nw.u[1] + c) & m1), fr * nw.w[1][c], fi * nw.w[1][c]);
+                atomic_add_c(row + (wrap_hi(nw.u[1] + c, n1)), fr * nw.w[1][c], fi * nw.w[1][c]);
             }
         }
     } else {
-        const int64_t n1 = ge.n[1], m1 = n1 - 1, n2 = ge.n[2], m2 = n2 - 1;
+        const int64_t n1 = ge.n[1], n2 = ge.n[2];
 #pragma unroll 1
         for (int a = 0; a < W; ++a) {
             T w0 = 0;
 #pragma unroll
             for (int q = 0; q < WS; ++q)
                 if (q == a) w0 = nw.w[0][q];
-            cplx_t<T> *plane = g + ((nw.u[0] + a) & m0) * n1 * n2;
+            cplx_t<T> *plane = g + (wrap_hi(nw.u[0] + a, n0)) * n1 * n2;
             const T fr0 = fj.x * w0, fi0 = fj.y * w0;
 #pragma unroll
             for (int c = 0; c < WS; ++c) {
                 if (WT == 0 && c >= W) break;
-                cplx_t<T> *row = plane + ((nw.u[1] + c) & m1) * n2;
+                cplx_t<T> *row = plane + (wrap_hi(nw.u[1] + c, n1)) * n2;
                 const T fr = fr0 * nw.w[1][c], fi = fi0 * nw.w[1][c];
 #pragma unroll
                 for (int e = 0; e < WS; ++e) {
                     if (WT == 0 && e >= W) break;
-                    atomic_add_c(row + ((nw.u[2] + e) & m2), fr * nw.w[2][e], fi * nw.w[2][e]);
+                    atomic_add_c(row + (wrap_hi(nw.u[2] + e, n2)), fr * nw.w[2][e], fi * nw.w[2][e]);
                 }
             }
         }
@@ -206,6 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_x(const cplx_t<T> *__restrict__
     for (int t = 0; t < D; ++t) {
         double c;
         window_start(x[j * D + t], ge.n[t], ge.m, u[t], c);
+        u[t] = wrap_lo(u[t], ge.n[t]);
         if constexpr (WS <= 18) {
             window_weights_dx<T, WS>(c, ge.invb[t], ge.nrm[t], ge.n[t], w[t], dw[t]);
         } else {
@@ -218,29 +220,29 @@ __global__ __launch_bounds__(kBlock) void k_grad_x(const cplx_t<T> *__restrict__
         }
     }
     const cplx_t<T> vj = v[j];
-    const int64_t m0 = ge.n[0] - 1;
+    const int64_t n0 = ge.n[0];
     if constexpr (D == 1) {
         T sr = 0, si = 0;
 #pragma unroll
         for (int a = 0; a < WS; ++a) {
             if (WT == 0 && a >= W) break;
-            const cplx_t<T> q = g[(u[0] + a) & m0];
+            const cplx_t<T> q = g[wrap_hi(u[0] + a, n0)];
             sr += q.x * dw[0][a];
             si += q.y * dw[0][a];
         }
         gx[j] = vj.x * sr + vj.y * si;
     } else if constexpr (D == 2) {
-        const int64_t n1 = ge.n[1], m1 = n1 - 1;
+        const int64_t n1 = ge.n[1];
         T s0r = 0, s0i = 0, s1r = 0, s1i = 0;
 #pragma unroll
         for (int a = 0; a < WS; ++a) {
             if (WT == 0 && a >= W) break;
-            const cplx_t<T> *row = g + ((u[0] + a) & m0) * n1;
+            const cplx_t<T> *row = g + (wrap_hi(u[0] + a, n0)) * n1;
             T ar = 0, ai = 0, br = 0, bi = 0;  // sum g w1, sum g dw1
 #pragma unroll
             for (int c = 0; c < WS; ++c) {
                 if (WT == 0 && c >= W) break;
-                const cplx_t<T> q = row[(u[1] + c) & m1];
+                const cplx_t<T> q = row[wrap_hi(u[1] + c, n1)];
                 ar += q.x * w[1][c];
                 ai += q.y * w[1][c];
                 br += q.x * dw[1][c];
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_x(const cplx_t<T> *__restrict__
         gx[2 * j] = vj.x * s0r + vj.y * s0i;
         gx[2 * j + 1] = vj.x * s1r + vj.y * s1i;
     } else {
-        const int64_t n1 = ge.n[1], m1 = n1 - 1, n2 = ge.n[2], m2 = n2 - 1;
+        const int64_t n1 = ge.n[1], n2 = ge.n[2];
         T s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
 #pragma unroll 1
         for (int a = 0; a < W; ++a) {
@@ -265,17 +267,17 @@ __global__ __launch_bounds__(kBlock) void k_grad_x(const cplx_t<T> *__restrict__
                     w0 = w[0][q];
                     dw0 = dw[0][q];
                 }
-            const cplx_t<T> *plane = g + ((u[0] + a) & m0) * n1 * n2;
+            const cplx_t<T> *plane = g + (wrap_hi(u[0] + a, n0)) * n1 * n2;
             T Ar = 0, Ai = 0, Br = 0, Bi = 0, Cr = 0, Ci = 0;
 #pragma unroll
             for (int c = 0; c < WS; ++c) {
                 if (WT == 0 && c >= W) break;
-                const cplx_t<T> *row = plane + ((u[1] + c) & m1) * n2;
+                const cplx_t<T> *row = plane + (wrap_hi(u[1] + c, n1)) * n2;
                 T ar = 0, ai = 0, br = 0, bi = 0;  // sum g w2, sum g dw2
 #pragma unroll
                 for (int e = 0; e < WS; ++e) {
                     if (WT == 0 && e >= W) break;
-                    const cplx_t<T> q = row[(u[2] + e) & m2];
+                    const cplx_t<T> q = row[wrap_hi(u[2] + e, n2)];
                     ar += q.x * w[2][e];
                     ai += q.y * w[2][e];
                     br += q.x * dw[2][e];

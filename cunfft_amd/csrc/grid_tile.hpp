@@ -401,8 +401,8 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
             o[t] = (rem % (uint32_t)bg.nt[t]) << bg.logT[t];
             rem /= (uint32_t)bg.nt[t];
         }
-        const uint32_t m0 = (uint32_t)ge.n[0] - 1, mL = (uint32_t)ge.n[D - 1] - 1;
-        const uint32_t m1 = D == 3 ? (uint32_t)ge.n[1] - 1 : 0;
+        const uint32_t n0 = (uint32_t)ge.n[0], nL = (uint32_t)ge.n[D - 1];
+        const uint32_t n1 = D == 3 ? (uint32_t)ge.n[1] : 1;
         const uint32_t s1 = D == 3 ? (uint32_t)ge.n[2] : 0;
         const uint32_t s0 = D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : (uint32_t)ge.n[1];
         VT *tv = reinterpret_cast<VT *>(tile);
@@ -410,8 +410,9 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
         for (int idx = tid; idx < L::P0 * L::P1 * NR; idx += NT) {
             const int row = idx / NR, un = idx - row * NR;
             const int p0 = row / L::P1, p1 = row - p0 * L::P1;
-            uint32_t cell = ((o[0] + p0) & m0) * s0 + ((o[D - 1] + CPL * un) & mL);
-            if (D == 3) cell += ((o[1] + p1) & m1) * s1;
+            // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
+            uint32_t cell = wrap_hi32(o[0] + p0, n0) * s0 + wrap_hi32(o[D - 1] + CPL * un, nL);
+            if (D == 3) cell += wrap_hi32(o[1] + p1, n1) * s1;
             tv[(row * L::RP) / CPL + un] = gv[cell / CPL];
         }
     }

@@ -20,6 +20,14 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define CUNFFT_HIP(call) ::cunfft::hip_check((call), #call)
 
+// FFT lengths this build runs: n = 2^a 3^b with a >= 1, n <= 4096.
+inline bool fft_length_ok(int64_t n)
+{
+    if (n < 2 || n > CUNFFT_MAX_AXIS || (n & 1)) return false;
+    while (n % 3 == 0) n /= 3;
+    return (n & (n - 1)) == 0;
+}
+
 // Geometry of one transform (window shape from sigma_t = n_t / N_t).
 Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m);
 // Device roll-off tables exp(fac_t k^2) by pruned index, offsets per axis.
@@ -59,7 +67,8 @@ template <typename T> struct FftPass {
     int64_t n[3];
     int64_t N[3];
     int d, t;
-    int L, logL;
+    int L, logL;             // logL = log2(L) for a power of two, else -1
+    int n3;                  // radix-3 factors of L (L = 2^a 3^n3)
     int64_t S;
     int64_t n_pa;            // number of lines over the axes before t
     int C, logC;             // lines per workgroup

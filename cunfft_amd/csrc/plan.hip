@@ -129,7 +129,6 @@ static int64_t next_pow2(int64_t v)
     return p;
 }
 
-static bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
 static int ilog2(int64_t v)
 {
@@ -148,8 +147,8 @@ template <typename T> static void validate(const typename Traits<T>::plan *p)
     for (int t = 0; t < p->d; ++t) {
         if (p->N[t] < 2 || (p->N[t] & 1))
             throw Error(CUNFFT_ERR_INVALID_ARG, "N_t must be even and >= 2");
-        if (!is_pow2(p->n[t]))
-            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t must be a power of two");
+        if (!fft_length_ok(p->n[t]))
+            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t must be 2^a 3^b with a >= 1 (and <= 4096)");
         if (p->n[t] < p->N[t]) throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= N_t");
         if (p->n[t] < 2 * p->m + 2)
             throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= 2m+2 (window wider than grid)");
@@ -941,7 +940,7 @@ template <typename T> void api_init_plan(int dim, uint_t *n, cunfftHandle *h)
     try {
         int64_t nn[3];
         for (int t = 0; t < dim; ++t) {
-            if (!is_pow2(n[t]) || n[t] > CUNFFT_MAX_AXIS) return;
+            if (!fft_length_ok(n[t])) return;
             nn[t] = n[t];
         }
         *h = make_fft_tables(sizeof(T) == 8 ? 64 : 32, dim, nn);

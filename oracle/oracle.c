@@ -186,7 +186,27 @@ static void fft_line(double *a, int64_t L, int sign, const double *tw)
     }
 }
 
-/* cufftExecZ2Z(..., CUFFT_FORWARD / CUFFT_INVERSE), src/cunfft_wrapper.cu:288-329. */
+/* Direct DFT of one line of any length L (the oracle's path for lengths that
+ * are not powers of two): out_k = sum_q a_q e^{sign 2 pi i q k / L}, the
+ * exponent reduced mod L before the table lookup. */
+static void dft_line(double *a, int64_t L, int sign, const double *tw, double *tmp)
+{
+    for (int64_t k = 0; k < L; ++k) {
+        double sr = 0.0, si = 0.0;
+        for (int64_t q = 0; q < L; ++q) {
+            const int64_t e = (q * k) % L;
+            const double wr = tw[2 * e], wi = sign * tw[2 * e + 1];
+            sr += a[2 * q] * wr - a[2 * q + 1] * wi;
+            si += a[2 * q] * wi + a[2 * q + 1] * wr;
+        }
+        tmp[2 * k] = sr;
+        tmp[2 * k + 1] = si;
+    }
+    for (int64_t k = 0; k < 2 * L; ++k) a[k] = tmp[k];
+}
+
+/* cufftExecZ2Z(..., CUFFT_FORWARD / CUFFT_INVERSE), src/cunfft_wrapper.cu:288-329.
+ * Power-of-two lengths use the radix-2 FFT, others the direct DFT. */
 void orc_fft(int d, const int64_t *n, double *g, int sign)
 {
     int64_t nt = prod(n, d);
@@ -203,7 +223,8 @@ void orc_fft(int d, const int64_t *n, double *g, int sign)
         /* sign convention: forward uses e^{-...}; fft_line uses e^{sign ...} */
 #pragma omp parallel
         {
-            double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)L);
+            double *buf = (double *)malloc(sizeof(double) * 4 * (size_t)L);
+            const int pow2 = (L & (L - 1)) == 0;
 #pragma omp for schedule(static)
             for (int64_t li = 0; li < lines; ++li) {
                 int64_t outer = li / stride, inner = li % stride;
@@ -212,7 +233,10 @@ void orc_fft(int d, const int64_t *n, double *g, int sign)
                     buf[2 * q] = g[2 * (base + q * stride)];
                     buf[2 * q + 1] = g[2 * (base + q * stride) + 1];
                 }
-                fft_line(buf, L, sign, tw);
+                if (pow2)
+                    fft_line(buf, L, sign, tw);
+                else
+                    dft_line(buf, L, sign, tw, buf + 2 * L);
                 for (int64_t q = 0; q < L; ++q) {
                     g[2 * (base + q * stride)] = buf[2 * q];
                     g[2 * (base + q * stride) + 1] = buf[2 * q + 1];

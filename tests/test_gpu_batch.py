@@ -176,7 +176,7 @@ def test_batch_rejects_bad_arguments(gpu):
     from cunfft_amd.batch import Batch
 
     with pytest.raises(gpu.CunfftError):
-        Batch([[30, 32]], [100], m=4, n=[[48, 64]])   # n not a power of two
+        Batch([[30, 32]], [100], m=4, n=[[40, 64]])   # n not of the form 2^a 3^b
     with pytest.raises(gpu.CunfftError):
         Batch([[31, 32]], [100], m=4)                  # odd N
     with pytest.raises(gpu.CunfftError):

@@ -82,7 +82,8 @@ def test_window_covers_neighbourhood(orc):
 
 
 # --------------------------------------------------------------------- FFT --
-@pytest.mark.parametrize("n", [[8], [64], [2048], [16, 32], [8, 4, 16]])
+@pytest.mark.parametrize("n", [[8], [64], [2048], [16, 32], [8, 4, 16],
+                               [6], [48], [192], [24, 96], [12, 6, 18]])
 def test_fft_matches_numpy(orc, n):
     g = crandn(rng(1), int(np.prod(n)))
     fw = orc.fft(n, g, -1)
@@ -213,6 +214,22 @@ def test_nfft_nonpow2_bandwidth(orc):
     x = r.uniform(-0.5, 0.5, (200, 2))
     fh = crandn(r, 240)
     assert orc.rel_l2(orc.ndft_trafo(N, x, fh), orc.nfft_trafo(N, n, m, x, fh)) < 1e-4
+
+
+@pytest.mark.parametrize("N,n,m,band", [([96], [192], 6, 1e-6), ([96, 64], [192, 128], 4, 3e-4),
+                                        ([24, 16, 24], [48, 32, 72], 4, 3e-4)])
+def test_nfft_on_grids_of_2a3b_cells(orc, N, n, m, band):
+    """n = 2^a 3^b (sigma = 2 exactly for N = 96): the NFFT stays within the
+    window band of the NDFT and keeps adjointness."""
+    r = rng(17)
+    x = r.uniform(-0.5, 0.5, (300, len(N)))
+    fh = crandn(r, int(np.prod(N)))
+    fv = crandn(r, 300)
+    assert orc.rel_l2(orc.ndft_trafo(N, x, fh), orc.nfft_trafo(N, n, m, x, fh)) < band
+    assert orc.rel_l2(orc.ndft_adjoint(N, x, fv), orc.nfft_adjoint(N, n, m, x, fv)) < band
+    lhs = np.vdot(fv, orc.nfft_trafo(N, n, m, x, fh))
+    rhs = np.vdot(orc.nfft_adjoint(N, n, m, x, fv), fh)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
 
 
 def test_edge_nodes(orc):

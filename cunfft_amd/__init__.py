@@ -151,6 +151,10 @@ def lib():
               "cunfftf_ndft_transform_rows_dev", "cunfftf_ndft_adjoint_cols_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, ctypes.c_longlong, vp, vp]
         getattr(L, s).restype = ctypes.c_int
+    for s in ("cunfft_solve_dev", "cunfftf_solve_dev"):
+        getattr(L, s).argtypes = [PP, vp, vp, vp, ctypes.c_int, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
+        getattr(L, s).restype = ctypes.c_int
     for s in ("cunfft_synchronize", "cunfftf_synchronize"):
         getattr(L, s).argtypes = [PP]
         getattr(L, s).restype = ctypes.c_int
@@ -399,6 +403,17 @@ class Plan:
         self._dev("cunfft_backward_dev", "cunfftf_backward_dev", ctypes.c_void_p(fhat_ptr),
                   ctypes.c_void_p(grad_f_ptr), ctypes.c_void_p(grad_fhat_ptr),
                   ctypes.c_void_p(grad_x_ptr), ctypes.c_void_p(stream))
+
+    def solve_dev(self, y_ptr, fhat_ptr, w_ptr=0, max_iter=50, tol=1e-10, stream=None):
+        """Inverse NFFT by CGNR (cunfft_solve_dev): fhat (initial guess, then the
+        solution) minimising sum_j w_j |y_j - (A fhat)_j|^2.  Returns (iterations,
+        relative residual per iteration)."""
+        hist = (ctypes.c_double * (max_iter + 1))()
+        it = ctypes.c_int(0)
+        self._dev("cunfft_solve_dev", "cunfftf_solve_dev", ctypes.c_void_p(y_ptr),
+                  ctypes.c_void_p(w_ptr), ctypes.c_void_p(fhat_ptr), ctypes.c_int(max_iter),
+                  ctypes.c_double(tol), hist, ctypes.byref(it), ctypes.c_void_p(stream))
+        return it.value, [hist[k] for k in range(it.value + 1)]
 
     def synchronize(self):
         """Wait for the plan's queued work (ASYNC_COPY plans; cunfft_synchronize)."""

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -115,6 +116,20 @@ template <typename T>
 void launch_grad_x(const Geom &ge, const cplx_t<T> *g, const double *x, const cplx_t<T> *v,
                    T *gx, hipStream_t s);
 template <typename T> void launch_zero(cplx_t<T> *p, int64_t n, hipStream_t s);
+
+// -------------------------------------------------------------- solver ---
+// The transforms the inverse NFFT iterates (device pointers, on stream s).
+template <typename T> struct SolverOps {
+    std::function<void(const cplx_t<T> *fhat, cplx_t<T> *f, hipStream_t s)> forward;
+    std::function<void(const cplx_t<T> *f, cplx_t<T> *fhat, hipStream_t s)> adjoint;
+};
+// CGNR for min_f sum_j w_j |y_j - (A f)_j|^2 (w NULL: all 1); f is the
+// initial guess and the result.  res_hist (may be NULL) receives
+// |y - A f|_W / |y|_W for iterations 0 .. *iters.
+template <typename T>
+void solve_cgnr(const SolverOps<T> &ops, const cplx_t<T> *y, const T *w, cplx_t<T> *f,
+                int64_t M, int64_t N, int max_iter, double tol, double *res_hist, int *iters,
+                hipStream_t s);
 
 // ---------------------------------------------------------------- NDFT ---
 template <typename T>

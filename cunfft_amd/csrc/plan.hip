@@ -927,6 +927,29 @@ template <typename T> int api_set_config(typename Traits<T>::plan *p, int dirs, 
     });
 }
 
+template <typename T>
+int api_solve_dev(typename Traits<T>::plan *p, const void *y, const T *w, void *fhat,
+                  int max_iter, double tol, double *res_hist, int *iters, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!y || !fhat) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        SolverOps<T> ops;
+        ops.forward = [&](const cplx_t<T> *in, cplx_t<T> *out, hipStream_t s) {
+            double a, b;
+            run_forward<T>(p, im, in, out, s, false, &a, &b);
+        };
+        ops.adjoint = [&](const cplx_t<T> *in, cplx_t<T> *out, hipStream_t s) {
+            double a, b, c;
+            run_adjoint<T>(p, im, in, out, s, false, &a, &b, &c);
+        };
+        solve_cgnr<T>(ops, static_cast<const cplx_t<T> *>(y), w, static_cast<cplx_t<T> *>(fhat),
+                      p->M_total, p->N_total, max_iter, tol, res_hist, iters,
+                      pick_stream(stream, im->stream));
+    });
+}
+
 template <typename T> int api_synchronize(typename Traits<T>::plan *p)
 {
     return guard(p, [&] { CUNFFT_HIP(hipStreamSynchronize(impl_of<T>(p)->stream)); });
@@ -1105,6 +1128,12 @@ int cunfft_autotune(cunfft_plan *p, int directions, int reps, float *ms, void *s
     return api_autotune<double>(p, directions, reps, ms, s);
 }
 int cunfft_synchronize(cunfft_plan *p) { return api_synchronize<double>(p); }
+int cunfft_solve_dev(cunfft_plan *p, const cunfftDoubleComplex *y, const double *w,
+                     cunfftDoubleComplex *fhat, int max_iter, double tol, double *res_hist,
+                     int *iters, void *s)
+{
+    return api_solve_dev<double>(p, y, w, fhat, max_iter, tol, res_hist, iters, s);
+}
 int cunfft_get_config(const cunfft_plan *p, int direction)
 {
     return api_get_config<double>(p, direction);
@@ -1223,6 +1252,12 @@ int cunfftf_autotune(cunfftf_plan *p, int directions, int reps, float *ms, void 
     return api_autotune<float>(p, directions, reps, ms, s);
 }
 int cunfftf_synchronize(cunfftf_plan *p) { return api_synchronize<float>(p); }
+int cunfftf_solve_dev(cunfftf_plan *p, const cunfftFloatComplex *y, const float *w,
+                      cunfftFloatComplex *fhat, int max_iter, double tol, double *res_hist,
+                      int *iters, void *s)
+{
+    return api_solve_dev<float>(p, y, w, fhat, max_iter, tol, res_hist, iters, s);
+}
 int cunfftf_get_config(const cunfftf_plan *p, int direction)
 {
     return api_get_config<float>(p, direction);

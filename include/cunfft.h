@@ -260,6 +260,23 @@ int cunfft_backward_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
                         double *grad_x_dev, void *stream);
 /* Wait for all work queued on the plan's stream (CUNFFT_ASYNC_COPY plans). */
 int cunfft_synchronize(cunfft_plan *plan);
+/* Inverse NFFT (no counterpart in the reference's code; its README:1 names
+ * it, and NFFT3's solver module, which the reference links for its CPU side,
+ * provides it): conjugate gradients on the normal equations (CGNR) for
+ *   min_fhat  sum_j w_j |y_j - (A fhat)_j|^2
+ * with the plan's nodes.  y: M_total complex; w: M_total weights or NULL
+ * (all 1, e.g. density compensation otherwise); fhat: the initial guess in,
+ * the solution out.  Stops after max_iter iterations, once
+ * |y - A fhat|_W <= tol |y|_W, or once the normal-equation residual
+ * |A^H W (y - A fhat)| falls to tol times its start (the least-squares
+ * solution of an inconsistent system) or grows tenfold over its smallest
+ * value (loss of orthogonality).  res_hist (NULL or max_iter + 1 doubles)
+ * receives the relative residual of each iteration, *iters (may be NULL)
+ * the iterations done.  Allocates its work vectors and frees them; runs on
+ * `stream` and synchronizes it. */
+int cunfft_solve_dev(cunfft_plan *plan, const cunfftDoubleComplex *y_dev, const double *w_dev,
+                     cunfftDoubleComplex *fhat_dev, int max_iter, double tol,
+                     double *res_hist, int *iters, void *stream);
 /* Gridding kernel configurations (no counterpart in the reference, whose
  * tile shape and kernels are fixed at compile time, src/cunfft_kernel.cu).
  * Configuration c < CUNFFT_TILE_SHAPES is the plane-tiled kernel pair (one
@@ -381,6 +398,9 @@ int cunfftf_backward_dev(cunfftf_plan *plan, const cunfftFloatComplex *fhat_dev,
                          const cunfftFloatComplex *grad_f_dev, cunfftFloatComplex *grad_fhat_dev,
                          float *grad_x_dev, void *stream);
 int cunfftf_synchronize(cunfftf_plan *plan);
+int cunfftf_solve_dev(cunfftf_plan *plan, const cunfftFloatComplex *y_dev, const float *w_dev,
+                      cunfftFloatComplex *fhat_dev, int max_iter, double tol,
+                      double *res_hist, int *iters, void *stream);
 int cunfftf_autotune(cunfftf_plan *plan, int directions, int reps, float *ms_out, void *stream);
 int cunfftf_get_config(const cunfftf_plan *plan, int direction);
 int cunfftf_set_config(cunfftf_plan *plan, int directions, int config);
@@ -427,6 +447,7 @@ typedef float dTyp;
 #define cunfft_grad_x_dev cunfftf_grad_x_dev
 #define cunfft_backward_dev cunfftf_backward_dev
 #define cunfft_synchronize cunfftf_synchronize
+#define cunfft_solve_dev cunfftf_solve_dev
 #define cunfft_autotune cunfftf_autotune
 #define cunfft_get_config cunfftf_get_config
 #define cunfft_set_config cunfftf_set_config

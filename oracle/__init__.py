@@ -47,6 +47,15 @@ def lib():
         L.orc_window_start.argtypes = [ctypes.c_int64, ctypes.c_double, ctypes.c_int]
         L.orc_phi.restype = ctypes.c_double
         L.orc_phi.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_set_window.argtypes = [ctypes.c_int]
+        L.orc_get_window.restype = ctypes.c_int
+        L.orc_kb_shape.restype = ctypes.c_double
+        L.orc_kb_shape.argtypes = [ctypes.c_int64, ctypes.c_int64]
+        for fn in (L.orc_phi_kb, L.orc_dphi_kb):
+            fn.restype = ctypes.c_double
+            fn.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_bessel_i0.restype = ctypes.c_double
+        L.orc_bessel_i0.argtypes = [ctypes.c_double]
         L.orc_rel_linf.restype = ctypes.c_double
         L.orc_rel_l2.restype = ctypes.c_double
         _lib = L
@@ -104,6 +113,45 @@ class Drand48:
         lib().orc_example_data_ad_unidistr(ctypes.byref(self.state), ctypes.c_int64(M),
                                            ctypes.c_int(d), _p(f.view(np.float64)), _p(x))
         return f, x
+
+
+WINDOWS = {"gaussian": 0, "kaiser_bessel": 1}
+
+
+def set_window(kind: str) -> None:
+    """Window of the oracle's transforms: "gaussian" (the reference's) or
+    "kaiser_bessel" (NFFT3's).  Process-wide; tests restore "gaussian"."""
+    lib().orc_set_window(WINDOWS[kind])
+
+
+class window:
+    """``with oracle.window("kaiser_bessel"): ...``"""
+
+    def __init__(self, kind):
+        self.kind = kind
+
+    def __enter__(self):
+        self.prev = lib().orc_get_window()
+        set_window(self.kind)
+
+    def __exit__(self, *exc):
+        lib().orc_set_window(self.prev)
+
+
+def kb_shape(N, n):
+    return lib().orc_kb_shape(int(N), int(n))
+
+
+def phi_kb(s, m, b):
+    return lib().orc_phi_kb(float(s), int(m), float(b))
+
+
+def dphi_kb(s, m, b):
+    return lib().orc_dphi_kb(float(s), int(m), float(b))
+
+
+def bessel_i0(z):
+    return lib().orc_bessel_i0(float(z))
 
 
 def shape_b(N, n, m):

@@ -78,6 +78,72 @@ double orc_phi(double t, double b)
     return exp(-(t * t) / b) / sqrt(M_PI * b);
 }
 
+/* ------------------------------------------------------ Kaiser-Bessel -- */
+/* The reference is Gaussian-only; the Kaiser-Bessel window restated here is
+ * NFFT3's (the library the reference links for its CPU side), in grid units
+ * s = n x - l with shape b = pi (2 - 1/sigma):
+ *   phi(s) = sinh(b sqrt(m^2 - s^2)) / (pi sqrt(m^2 - s^2))   |s| < m
+ *          = b / pi                                          |s| = m
+ *          = sin(b sqrt(s^2 - m^2)) / (pi sqrt(s^2 - m^2))    |s| > m
+ * and its Fourier transform at xi = k/n, phihat = I0(m sqrt(b^2 - (2 pi k/n)^2)).
+ * Both branches are one power series in d = m^2 - s^2,
+ *   phi = (1/pi) sum_j b^{2j+1} d^j / (2j+1)!,
+ * used near d = 0. */
+static int g_window = 0;   /* 0 Gaussian, 1 Kaiser-Bessel */
+
+void orc_set_window(int kind) { g_window = kind; }
+int orc_get_window(void) { return g_window; }
+
+double orc_kb_shape(int64_t N, int64_t n) { return M_PI * (2.0 - (double)N / (double)n); }
+
+double orc_phi_kb(double s, int m, double b)
+{
+    const double d = (double)m * m - s * s;
+    if (fabs(d) < 1e-8) return (b + b * b * b * d / 6.0) / M_PI;
+    const double r = sqrt(fabs(d));
+    return (d > 0 ? sinh(b * r) : sin(b * r)) / (M_PI * r);
+}
+
+double orc_dphi_kb(double s, int m, double b)
+{
+    const double d = (double)m * m - s * s;
+    if (fabs(d) < 1e-4) {
+        const double b2 = b * b;
+        /* d/dd of the series, times dd/ds = -2 s */
+        const double g = b * b2 / 6.0 + 2.0 * b * b2 * b2 * d / 120.0 +
+                         3.0 * b * b2 * b2 * b2 * d * d / 5040.0 +
+                         4.0 * b * b2 * b2 * b2 * b2 * d * d * d / 362880.0;
+        return -2.0 * s * g / M_PI;
+    }
+    const double r = sqrt(fabs(d));
+    const double br = b * r;
+    if (d > 0) return -s * (br * cosh(br) - sinh(br)) / (M_PI * r * r * r);
+    return s * (br * cos(br) - sin(br)) / (M_PI * r * r * r);
+}
+
+/* Modified Bessel I0 by its power series in long double. */
+double orc_bessel_i0(double z)
+{
+    long double q = (long double)z * z / 4.0L, term = 1.0L, sum = 1.0L;
+    for (int j = 1; j < 500; ++j) {
+        term *= q / ((long double)j * j);
+        sum += term;
+        if (term < 1e-22L * sum) break;
+    }
+    return (double)sum;
+}
+
+/* Roll-off factor 1 / phihat(k / n) of the current window for one axis. */
+static double rolloff_factor(int64_t N, int64_t n, int m, int64_t k)
+{
+    if (g_window == 1) {
+        const double b = orc_kb_shape(N, n), w = 2.0 * M_PI * (double)k / (double)n;
+        return 1.0 / orc_bessel_i0((double)m * sqrt(b * b - w * w));
+    }
+    const double b = orc_shape_b(N, n, m);
+    return exp(M_PI * M_PI * b / ((double)n * (double)n) * (double)k * (double)k);
+}
+
 static int64_t pmod(int64_t a, int64_t n)
 {
     int64_t r = a % n;
@@ -100,15 +166,10 @@ void orc_rolloff_pad(int d, const int64_t *N, const int64_t *n, int m,
                      const double *fhat, double *ghat)
 {
     int64_t Nt = prod(N, d), nt = prod(n, d);
-    double fac[MAXD];
-    for (int t = 0; t < d; ++t) {
-        double b = orc_shape_b(N[t], n[t], m);
-        fac[t] = M_PI * M_PI * b / ((double)n[t] * (double)n[t]);
-    }
     memset(ghat, 0, sizeof(double) * 2 * (size_t)nt);
     for (int64_t kl = 0; kl < Nt; ++kl) {
         int64_t rem = kl, gidx = 0, gstride = 1;
-        double e = 0.0;
+        double s = 1.0;
         int64_t k[MAXD];
         for (int t = d - 1; t >= 0; --t) {
             k[t] = rem % N[t] - N[t] / 2;
@@ -117,9 +178,8 @@ void orc_rolloff_pad(int d, const int64_t *N, const int64_t *n, int m,
         for (int t = d - 1; t >= 0; --t) {
             gidx += pmod(k[t], n[t]) * gstride;
             gstride *= n[t];
-            e += fac[t] * (double)k[t] * (double)k[t];
+            s *= rolloff_factor(N[t], n[t], m, k[t]);
         }
-        double s = exp(e);
         ghat[2 * gidx] = fhat[2 * kl] * s;
         ghat[2 * gidx + 1] = fhat[2 * kl + 1] * s;
     }
@@ -131,14 +191,9 @@ void orc_crop_rolloff(int d, const int64_t *N, const int64_t *n, int m,
                       const double *g, double *fhat)
 {
     int64_t Nt = prod(N, d);
-    double fac[MAXD];
-    for (int t = 0; t < d; ++t) {
-        double b = orc_shape_b(N[t], n[t], m);
-        fac[t] = M_PI * M_PI * b / ((double)n[t] * (double)n[t]);
-    }
     for (int64_t kl = 0; kl < Nt; ++kl) {
         int64_t rem = kl, gidx = 0, gstride = 1;
-        double e = 0.0;
+        double s = 1.0;
         int64_t k[MAXD];
         for (int t = d - 1; t >= 0; --t) {
             k[t] = rem % N[t] - N[t] / 2;
@@ -147,9 +202,8 @@ void orc_crop_rolloff(int d, const int64_t *N, const int64_t *n, int m,
         for (int t = d - 1; t >= 0; --t) {
             gidx += pmod(k[t], n[t]) * gstride;
             gstride *= n[t];
-            e += fac[t] * (double)k[t] * (double)k[t];
+            s *= rolloff_factor(N[t], n[t], m, k[t]);
         }
-        double s = exp(e);
         fhat[2 * kl] = g[2 * gidx] * s;
         fhat[2 * kl + 1] = g[2 * gidx + 1] * s;
     }
@@ -262,15 +316,21 @@ static void node_window(int d, const int64_t *N, const int64_t *n, int m,
     int W = 2 * m + 2;
     for (int t = 0; t < d; ++t) {
         double b = orc_shape_b(N[t], n[t], m);
+        double bk = orc_kb_shape(N[t], n[t]);
         int64_t u = orc_window_start(n[t], xj[t], m);
         double nx = (double)n[t] * xj[t];
         for (int l = 0; l < W; ++l) {
             double s = nx - (double)(u + l);
-            double p = orc_phi(s, b);
             nw->idx[t][l] = pmod(u + l, n[t]);
-            nw->w[t][l] = p;
-            /* d/dx phi(n x - l) = n * (-2 s / b) phi(s) */
-            if (want_dw) nw->dw[t][l] = (double)n[t] * (-2.0 * s / b) * p;
+            if (g_window == 1) {
+                nw->w[t][l] = orc_phi_kb(s, m, bk);
+                if (want_dw) nw->dw[t][l] = (double)n[t] * orc_dphi_kb(s, m, bk);
+            } else {
+                double p = orc_phi(s, b);
+                nw->w[t][l] = p;
+                /* d/dx phi(n x - l) = n * (-2 s / b) phi(s) */
+                if (want_dw) nw->dw[t][l] = (double)n[t] * (-2.0 * s / b) * p;
+            }
         }
     }
 }

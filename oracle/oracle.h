@@ -51,6 +51,18 @@ int64_t orc_window_start(int64_t n, double x, int m);
 /* Gaussian window in grid units (src/cunfft_kernel.cu:162-166). */
 double orc_phi(double t, double b);
 
+/* Window of the transforms below: 0 Gaussian (the reference's, default),
+ * 1 Kaiser-Bessel (NFFT3's, which the reference links for its CPU side;
+ * no counterpart in the reference's GPU code).  Process-wide setting. */
+void orc_set_window(int kind);
+int orc_get_window(void);
+/* Kaiser-Bessel shape b = pi (2 - 1/sigma), window phi(s) in grid units,
+ * d phi / d s, and I0 (power series). */
+double orc_kb_shape(int64_t N, int64_t n);
+double orc_phi_kb(double s, int m, double b);
+double orc_dphi_kb(double s, int m, double b);
+double orc_bessel_i0(double z);
+
 /* Forward roll-off + zero pad: fhat (N_total) -> ghat (n_total), zeros
  * outside the centred box (src/cunfft_wrapper.cu:32-75,
  * src/cunfft_kernel.cu:240-412). */
@@ -61,7 +73,8 @@ void orc_rolloff_pad(int d, const int64_t *N, const int64_t *n, int m,
 void orc_crop_rolloff(int d, const int64_t *N, const int64_t *n, int m,
                       const double *g, double *fhat);
 /* In-place unnormalised d-dim FFT, sign -1 forward / +1 adjoint
- * (cufftExecZ2Z at src/cunfft_wrapper.cu:288-329). Power-of-two axes. */
+ * (cufftExecZ2Z at src/cunfft_wrapper.cu:288-329): radix 2 on power-of-two
+ * axes, a direct DFT on the others. */
 void orc_fft(int d, const int64_t *n, double *g, int sign);
 /* Forward gridding (gather), f_j = sum_l g_l prod_t phi(n_t x_jt - l_t)
  * (src/cunfft_kernel.cu:514-1093). */

@@ -362,3 +362,69 @@ def test_nfft_dx_near_ndft_dx(orc):
     a = orc.nfft_dx(N, n, m, x, fh)
     b = orc.ndft_dx(N, x, fh)
     assert orc.rel_l2(b.reshape(-1), a.reshape(-1)) < 1e-4
+
+
+# ------------------------------------------------------- Kaiser-Bessel ----
+def test_bessel_i0_matches_scipy(orc):
+    from scipy.special import i0
+
+    for z in [0.0, 1e-3, 0.5, 3.0, 9.42, 18.85, 40.0, 75.4]:
+        assert abs(orc.bessel_i0(z) / i0(z) - 1.0) <= 1e-14
+
+
+def test_kaiser_bessel_window_shape(orc):
+    m, b = 4, orc.kb_shape(64, 128)
+    assert abs(b - 1.5 * np.pi) <= 1e-15
+    # symmetric, continuous through |s| = m (value b / pi), series branch agrees
+    for s in [0.3, 1.7, 3.9]:
+        assert orc.phi_kb(s, m, b) == orc.phi_kb(-s, m, b)
+    for eps in [1e-3, 1e-5, 1e-7, 1e-9]:
+        # |phi'(m)| = b^3 m / (3 pi) ~ 44 here: the value moves linearly through b / pi
+        assert abs(orc.phi_kb(m - eps, m, b) - b / np.pi) <= 50 * eps + 1e-12
+        assert abs(orc.phi_kb(m + eps, m, b) - b / np.pi) <= 50 * eps + 1e-12
+    # phi(0) closed form, derivative by central differences (both branches, and near m)
+    assert abs(orc.phi_kb(0.0, m, b) - np.sinh(b * m) / (np.pi * m)) <= 1e-14 * np.sinh(b * m)
+    for s in [0.0, 0.8, 2.5, 3.99, 3.99999, 4.00001, 4.3, 5.2]:
+        h = 1e-6
+        fd = (orc.phi_kb(s + h, m, b) - orc.phi_kb(s - h, m, b)) / (2 * h)
+        an = orc.dphi_kb(s, m, b)
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)) + 1e-6 * orc.phi_kb(0.0, m, b) * 1e-6
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_kaiser_bessel_nfft_converges_to_the_ndft(orc, d):
+    """The Kaiser-Bessel NFFT reaches the NDFT far faster in m than the
+    Gaussian (a wrong shape, roll-off or normalisation breaks this)."""
+    N = {1: [64], 2: [32, 32], 3: [16, 16, 16]}[d]
+    n = [2 * v for v in N]
+    r = rng(30 + d)
+    x = r.uniform(-0.5, 0.5, (300, d))
+    fh = crandn(r, int(np.prod(N)))
+    fv = crandn(r, 300)
+    nd, na = orc.ndft_trafo(N, x, fh), orc.ndft_adjoint(N, x, fv)
+    bands = {2: 1e-3, 4: 3e-7, 6: 1e-10, 8: 1e-13}
+    with orc.window("kaiser_bessel"):
+        for m, band in bands.items():
+            assert orc.rel_l2(nd, orc.nfft_trafo(N, n, m, x, fh)) <= band, m
+            assert orc.rel_l2(na, orc.nfft_adjoint(N, n, m, x, fv)) <= band, m
+        lhs = np.vdot(fv, orc.nfft_trafo(N, n, 4, x, fh))
+        rhs = np.vdot(orc.nfft_adjoint(N, n, 4, x, fv), fh)
+        assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    # the Gaussian at the same m is far less accurate
+    assert orc.rel_l2(nd, orc.nfft_trafo(N, n, 4, x, fh)) > 1e-5
+
+
+def test_kaiser_bessel_gradient_matches_differences(orc):
+    N, n, m = [16, 16], [32, 32], 4
+    r = rng(40)
+    x = r.uniform(-0.4, 0.4, (20, 2))
+    fh = crandn(r, 256)
+    with orc.window("kaiser_bessel"):
+        dfdx = orc.nfft_dx(N, n, m, x, fh)
+        h = 1e-7
+        for j, t in [(0, 0), (3, 1), (7, 0), (11, 1)]:
+            xp, xm = x.copy(), x.copy()
+            xp[j, t] += h
+            xm[j, t] -= h
+            fd = (orc.nfft_trafo(N, n, m, xp, fh)[j] - orc.nfft_trafo(N, n, m, xm, fh)[j]) / (2 * h)
+            assert abs(fd - dfdx[j, t]) <= 1e-6 * abs(dfdx[j, t]) + 1e-8

@@ -36,9 +36,10 @@ class NFFTOperator:
     passed to each call (and re-binned on the plan's first transform after).
     """
 
-    def __init__(self, N, M, n=None, m=4, precision="double"):
+    def __init__(self, N, M, n=None, m=4, precision="double", flags=0):
+        """flags: extra plan flags (e.g. KAISER_BESSEL, COMPENSATED)."""
         self.plan = Plan(N, M, n=n, m=m, precision=precision,
-                         flags=CUFFT_INIT | CUFFT_BOTH | NO_HOST_BUFFERS)
+                         flags=CUFFT_INIT | CUFFT_BOTH | NO_HOST_BUFFERS | flags)
         self.N = [int(v) for v in np.atleast_1d(N)]
         self.d = len(self.N)
         self.M = int(M)

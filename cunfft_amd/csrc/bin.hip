@@ -349,7 +349,7 @@ __device__ __forceinline__ int node_local(const double *__restrict__ xs, uint32_
         window_start(xs[(int64_t)s * D + t], ge.n[t], ge.m, u, c);
         int lo = (int)(wrap_lo(u, ge.n[t]) - o[t]);
         lo = lo < 0 ? 0 : (lo >= TileShape<D>::T[t] ? TileShape<D>::T[t] - 1 : lo);
-        window_weights<T, W>(c, ge.invb[t], ge.nrm[t], w[t]);
+        window_weights<T, W>(c, ge, t, w[t]);
         base += lo * stride[t];
     }
     return base;

@@ -106,9 +106,11 @@ BinGeom bin_geom(const Binning &b);
 
 // Window constants of the plane-tiled kernels.
 template <typename T> struct WinConst {
-    T kb[3];    // FP32: log2(e) / b_t (for exp2); FP64: 1 / b_t (for exp)
-    T nrm[3];   // (pi b_t)^-1/2
+    T kb[3];    // Gaussian: FP32 log2(e) / b_t (for exp2); FP64 1 / b_t (for exp)
+    T nrm[3];   // Gaussian: (pi b_t)^-1/2
+    T kbb[3];   // Kaiser-Bessel shape b_t
     int m;
+    int win;    // Window
 };
 template <typename T> WinConst<T> win_const(const Geom &ge);
 

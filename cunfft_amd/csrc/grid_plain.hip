@@ -38,12 +38,9 @@ __device__ __forceinline__ void load_window(const Geom &ge, const double *__rest
         window_start(x[j * D + t], ge.n[t], ge.m, nw.u[t], c);
         nw.u[t] = wrap_lo(nw.u[t], ge.n[t]);
         if constexpr (WS <= 18) {
-            window_weights<T, WS>(c, ge.invb[t], ge.nrm[t], nw.w[t]);
+            window_weights<T, WS>(c, ge, t, nw.w[t]);
         } else {
-            for (int l = 0; l < W; ++l) {
-                const T s = (T)(c - (double)l);
-                nw.w[t][l] = win_exp<T>(-s * s * (T)ge.invb[t]) * (T)ge.nrm[t];
-            }
+            for (int l = 0; l < W; ++l) nw.w[t][l] = window_weight<T>((T)(c - (double)l), ge, t);
         }
     }
 }
@@ -209,13 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_x(const cplx_t<T> *__restrict__
         window_start(x[j * D + t], ge.n[t], ge.m, u[t], c);
         u[t] = wrap_lo(u[t], ge.n[t]);
         if constexpr (WS <= 18) {
-            window_weights_dx<T, WS>(c, ge.invb[t], ge.nrm[t], ge.n[t], w[t], dw[t]);
+            window_weights_dx<T, WS>(c, ge, t, w[t], dw[t]);
         } else {
             for (int l = 0; l < W; ++l) {
                 const T s = (T)(c - (double)l);
-                const T p = win_exp<T>(-s * s * (T)ge.invb[t]) * (T)ge.nrm[t];
-                w[t][l] = p;
-                dw[t][l] = (T)ge.n[t] * ((T)-2.0 * s * (T)ge.invb[t]) * p;
+                w[t][l] = window_weight<T>(s, ge, t);
+                dw[t][l] = window_weight_dx<T>(s, w[t][l], ge, t);
             }
         }
     }

@@ -173,6 +173,13 @@ template <typename T, int D, int W, int PS, int V> struct TileLayout {
 
 // ----------------------------------------------------------- helpers ----
 
+// The Kaiser-Bessel weight out of line: the Gaussian path keeps its
+// registers (the window is a run-time choice of the plan).
+template <typename T> __device__ __noinline__ T kb_phi_call(T s, T m, T b)
+{
+    return kb_phi<T>(s, m, b);
+}
+
 template <typename T> __device__ __forceinline__ T wexp(T v);
 template <> __device__ __forceinline__ float wexp<float>(float v)
 {
@@ -310,7 +317,9 @@ __device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB> &wb, int
 #pragma unroll
         for (int ps = 0; ps < NPASS; ++ps) {
             if (lane + 64 * ps >= nb * W) break;
-            const T w = wc.nrm[t] * wexp<T>(-wc.kb[t] * sv[ps] * sv[ps]);
+            const T w = wc.win == WIN_KAISER_BESSEL
+                            ? kb_phi_call<T>(sv[ps], (T)wc.m, wc.kbb[t])
+                            : wc.nrm[t] * wexp<T>(-wc.kb[t] * sv[ps] * sv[ps]);
             if (t == 0) wb.w0[nv[ps]][lv[ps]] = w;
             else if (t == D - 1) wb.w2[nv[ps]][lv[ps] + 1] = w;
             else wb.w1[nv[ps]][lv[ps]] = w;
@@ -927,8 +936,10 @@ template <typename T> WinConst<T> win_const(const Geom &ge)
     for (int t = 0; t < 3; ++t) {
         w.kb[t] = (T)(ge.invb[t] * sc);
         w.nrm[t] = (T)ge.nrm[t];
+        w.kbb[t] = (T)ge.kbb[t];
     }
     w.m = ge.m;
+    w.win = ge.win;
     return w;
 }
 

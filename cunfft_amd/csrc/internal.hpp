@@ -30,8 +30,11 @@ inline bool fft_length_ok(int64_t n)
 }
 
 // Geometry of one transform (window shape from sigma_t = n_t / N_t).
-Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m);
-// Device roll-off tables exp(fac_t k^2) by pruned index, offsets per axis.
+Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m,
+               int win = WIN_GAUSSIAN);
+// Device roll-off tables 1 / phihat_t(k / n_t) by pruned index (Gaussian:
+// exp(fac_t k^2); Kaiser-Bessel: 1 / I0(m sqrt(b^2 - (2 pi k / n)^2))),
+// offsets per axis.
 double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3]);
 
 // ----------------------------------------------------------------- FFT ---

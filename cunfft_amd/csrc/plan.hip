@@ -192,13 +192,14 @@ template <typename T> static void init_device_para(typename Traits<T>::plan *p)
 
 // Window shape b_t = 2 sigma_t / (2 sigma_t - 1) * m / pi with sigma_t = n_t / N_t
 // (src/cunfft.cu:351-357) and the gridding geometry of one transform.
-Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m)
+Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m, int win)
 {
     Geom ge{};
     ge.d = d;
     ge.m = m;
     ge.W = 2 * m + 2;
     ge.M = M;
+    ge.win = win;
     for (int t = 0; t < 3; ++t) {
         ge.N[t] = t < d ? N[t] : 1;
         ge.n[t] = t < d ? n[t] : 1;
@@ -207,11 +208,26 @@ Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m)
             const double b = (2.0 * sigma) / (2.0 * sigma - 1.0) * ((double)m / M_PI);
             ge.invb[t] = 1.0 / b;
             ge.nrm[t] = 1.0 / sqrt(M_PI * b);
+            // Kaiser-Bessel shape of NFFT3: b = pi (2 - 1 / sigma)
+            ge.kbb[t] = M_PI * (2.0 - 1.0 / sigma);
         } else {
-            ge.invb[t] = ge.nrm[t] = 0.0;
+            ge.invb[t] = ge.nrm[t] = ge.kbb[t] = 0.0;
         }
     }
     return ge;
+}
+
+// Modified Bessel I0 by its power series in long double (z <= m pi 2 < 101).
+static double bessel_i0(double z)
+{
+    const long double q = (long double)z * z / 4.0L;
+    long double term = 1.0L, sum = 1.0L;
+    for (int j = 1; j < 1000; ++j) {
+        term *= q / ((long double)j * j);
+        sum += term;
+        if (term < 1e-22L * sum) break;
+    }
+    return (double)sum;
 }
 
 // Device roll-off tables by pruned index q < N_t: exp(fac_t k^2) with
@@ -228,7 +244,13 @@ double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3])
         const int64_t Nn = ge.N[t];
         for (int64_t q = 0; q < Nn; ++q) {
             const int64_t k = q < Nn / 2 ? q : q - Nn;
-            roc.push_back(exp(fac * (double)k * (double)k));
+            if (ge.win == WIN_KAISER_BESSEL) {
+                const double w = 2.0 * M_PI * (double)k / (double)ge.n[t];
+                const double z = (double)ge.m * sqrt(ge.kbb[t] * ge.kbb[t] - w * w);
+                roc.push_back(1.0 / bessel_i0(z));
+            } else {
+                roc.push_back(exp(fac * (double)k * (double)k));
+            }
         }
     }
     double *dev = nullptr;
@@ -325,7 +347,8 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
         NN[t] = p->N[t];
         nn3[t] = p->n[t];
     }
-    im->geom = make_geom(d, NN, nn3, p->M_total, p->m);
+    im->geom = make_geom(d, NN, nn3, p->M_total, p->m,
+                         (p->flags & CUNFFT_KAISER_BESSEL) ? WIN_KAISER_BESSEL : WIN_GAUSSIAN);
     const int c0 = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
     im->compensated = (p->flags & CUNFFT_COMPENSATED) && sizeof(T) == 4;
     plan_configs<T>(im, c0, c0);

@@ -78,6 +78,11 @@ extern "C" {
  *  must not change plan->x / f_hat / f until the plan's next copy to the
  *  host or cunfft_synchronize.  CUNFFT_MEASURE_TIME synchronizes anyway. */
 #define CUNFFT_ASYNC_COPY     (1U << 14)
+/** Kaiser-Bessel window instead of the reference's Gaussian (NFFT3's
+ *  window, b = pi (2 - 1/sigma), roll-off 1 / I0(m sqrt(b^2 - (2 pi k/n)^2))):
+ *  the same 2m+2 taps per axis reach ~1e-7 at m = 4 and ~1e-14 at m = 8
+ *  against the NDFT, where the Gaussian reaches ~1e-4 and ~1e-8. */
+#define CUNFFT_KAISER_BESSEL  (1U << 15)
 
 /* status codes */
 #define CUNFFT_SUCCESS          0

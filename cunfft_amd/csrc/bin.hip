@@ -367,8 +367,19 @@ __device__ __forceinline__ void gather_tiled(const cplx_t<T> *__restrict__ g,
     __shared__ cplx_t<T> tile[PD::PT];
     int64_t o[3];
     tile_origin<D>(ch.tile, bg, o);
-    for (int idx = threadIdx.x; idx < PD::PT; idx += kTileThreads)
-        tile[idx] = g[tile_cell<D, W>(idx, o, ge)];
+    // 8 loads in flight per thread before the LDS stores (unconditional: a
+    // lane past the end rewrites the last cell with its own value)
+    for (int i0 = threadIdx.x; i0 < PD::PT; i0 += 8 * kTileThreads) {
+        cplx_t<T> v[8];
+        int at[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            at[u] = i0 + u * kTileThreads < PD::PT ? i0 + u * kTileThreads : PD::PT - 1;
+            v[u] = g[tile_cell<D, W>(at[u], o, ge)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) tile[at[u]] = v[u];
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < ch.count; i += kTileThreads) {
         const uint32_t s = ch.start + i;

@@ -49,6 +49,7 @@ BinGeom bin_geom(const Binning &b)
         g.nt[t] = b.nt[t];
     }
     g.ntiles = b.ntiles;
+    g.sub_shift = b.elem_bytes == 4 ? 1 : 0;
     return g;
 }
 
@@ -61,20 +62,25 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, int64_t n, uint32
 
 __global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
                                                   uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ sub,
                                                   uint32_t *__restrict__ tile_count, Geom ge,
                                                   BinGeom bg)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= ge.M) return;
     uint32_t key = 0;
+    uint32_t lo[3] = {0, 0, 0};
     for (int t = 0; t < bg.d; ++t) {
         int64_t u;
         double c;
         window_start(x[j * bg.d + t], ge.n[t], ge.m, u, c);
-        const uint32_t tc = (uint32_t)(wrap_lo(u, ge.n[t]) >> bg.logT[t]);
+        const int64_t w = wrap_lo(u, ge.n[t]);
+        const uint32_t tc = (uint32_t)(w >> bg.logT[t]);
+        lo[t] = (uint32_t)(w - ((int64_t)tc << bg.logT[t]));
         key = key * (uint32_t)bg.nt[t] + tc;
     }
     keys[j] = key;
+    if (sub) sub[j] = sub_key(bg.d, lo, bg.logT, bg.sub_shift);
     atomicAdd(&tile_count[key], 1u);
 }
 
@@ -125,15 +131,61 @@ __global__ __launch_bounds__(256) void k_bin_place(const double *__restrict__ x,
                                                    const uint32_t *__restrict__ start,
                                                    uint32_t *__restrict__ cursor,
                                                    uint32_t *__restrict__ perm,
-                                                   double *__restrict__ xs, int64_t M, int d)
+                                                   double *__restrict__ xs,
+                                                   const uint32_t *__restrict__ sub,
+                                                   uint32_t *__restrict__ sub_out, int64_t M, int d)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= M) return;
     const uint32_t k = keys[j];
     const uint32_t pos = start[k] + atomicAdd(&cursor[k], 1u);
     perm[pos] = (uint32_t)j;
+    if (sub) sub_out[pos] = sub[j];
     if (xs)
         for (int t = 0; t < d; ++t) xs[(int64_t)pos * d + t] = x[j * d + t];
+}
+
+// Second pass of the sort, one wavefront per tile: a counting sort of the
+// tile's nodes on sub_key (LDS histogram of nrange bins, wavefront scan,
+// placement), from the placement order perm_in (keys sub_in, in the same
+// order) to perm_out.  Equal keys keep no particular order, as in the
+// placement.  nrange is a power of two >= 64.
+__global__ __launch_bounds__(64) void k_bin_subsort(const uint32_t *__restrict__ cnt,
+                                                    const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ sub_in,
+                                                    const uint32_t *__restrict__ perm_in,
+                                                    uint32_t *__restrict__ perm_out,
+                                                    uint32_t nrange)
+{
+    extern __shared__ uint32_t hist[];
+    const uint32_t tile = blockIdx.x, lane = threadIdx.x;
+    const uint32_t c = cnt[tile], s0 = start[tile];
+    if (c <= 1) {
+        if (c == 1 && lane == 0) perm_out[s0] = perm_in[s0];
+        return;
+    }
+    for (uint32_t i = lane; i < nrange; i += 64) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[sub_in[s0 + i]], 1u);
+    __syncthreads();
+    // exclusive scan: lane owns bins [lane * per, (lane + 1) * per)
+    const uint32_t per = nrange / 64, b0 = lane * per;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < per; ++i) sum += hist[b0 + i];
+    uint32_t inc = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(inc, off, 64);
+        if ((int)lane >= off) inc += v;
+    }
+    uint32_t run = inc - sum;
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint32_t h = hist[b0 + i];
+        hist[b0 + i] = run;
+        run += h;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < c; i += 64)
+        perm_out[s0 + atomicAdd(&hist[sub_in[s0 + i]], 1u)] = perm_in[s0 + i];
 }
 
 // Window descriptor of every sorted node (NodeDesc).  The index/fraction split
@@ -243,6 +295,11 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
         max_chunks = ntiles + (M + kChunk - 1) / kChunk;
         keys = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
         perm = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+        if (planes()) {
+            sub = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+            sub2 = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+            perm2 = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+        }
         tile_count = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
         tile_start = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
         cursor = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
@@ -263,12 +320,18 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
     const BinGeom bg = bin_geom(*this);
     const unsigned gm = (unsigned)((M + 255) / 256);
     k_fill_u32<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, ntiles, 0u);
-    k_bin_keys<<<gm, 256, 0, s>>>(x, keys, tile_count, ge, bg);
+    k_bin_keys<<<gm, 256, 0, s>>>(x, keys, planes() ? sub : nullptr, tile_count, ge, bg);
     k_bin_scan<<<1, 1024, 0, s>>>(tile_count, tile_start, cursor, chunk_start, n_chunks, ntiles);
-    k_bin_place<<<gm, 256, 0, s>>>(x, keys, tile_start, cursor, perm, xs, M, d);
+    k_bin_place<<<gm, 256, 0, s>>>(x, keys, tile_start, cursor, planes() ? perm2 : perm, xs,
+                                   planes() ? sub : nullptr, sub2, M, d);
     k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
                                                                   chunk_start, chunks, ntiles);
     if (planes()) {
+        const uint32_t nr = sub_range(bg);
+        if (nr < 64 || (nr & (nr - 1)) != 0 || nr > 4096)
+            throw Error(CUNFFT_ERR_UNSUPPORTED, "tile sub-key range must be a power of two in [64, 4096]");
+        k_bin_subsort<<<(unsigned)ntiles, 64, nr * sizeof(uint32_t), s>>>(
+            tile_count, tile_start, sub2, perm2, perm, nr);
         if (elem_bytes == 4)
             k_bin_desc<float><<<gm, 256, 0, s>>>(x, perm, keys, (NodeDesc<float> *)desc, ge, bg);
         else
@@ -281,7 +344,10 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
 void Binning::release()
 {
     if (keys) (void)hipFree(keys);
+    if (sub) (void)hipFree(sub);
+    if (sub2) (void)hipFree(sub2);
     if (perm) (void)hipFree(perm);
+    if (perm2) (void)hipFree(perm2);
     if (tile_count) (void)hipFree(tile_count);
     if (tile_start) (void)hipFree(tile_start);
     if (cursor) (void)hipFree(cursor);
@@ -292,7 +358,7 @@ void Binning::release()
     if (desc) (void)hipFree(desc);
     if (slots) (void)hipFree(slots);
     if (fs) (void)hipFree(fs);
-    keys = perm = tile_count = tile_start = cursor = chunk_start = n_chunks = nullptr;
+    keys = sub = sub2 = perm = perm2 = tile_count = tile_start = cursor = chunk_start = n_chunks = nullptr;
     chunks = nullptr;
     xs = nullptr;
     desc = nullptr;

@@ -60,7 +60,11 @@ template <> struct alignas(16) NodeDesc<double> {
 // start of their window, u_t mod n_t.  Built once per node set on the plan
 // stream and reused by every forward and adjoint call until the nodes change.
 // Tiles with many nodes are split into chunks so one workgroup never owns more
-// than kChunk nodes.
+// than kChunk nodes.  For the plane-tiled kernels the nodes of a tile are
+// further sorted by their window start within it: row and column unit first,
+// plane last (sub_key), so nodes whose windows cover the same rows and
+// column units follow each other with rising plane; the adjoint accumulates
+// such a run in registers and writes the tile once per run.
 struct Binning {
     int built = 0;
     int usable = 0;             // tiled kernels apply to this geometry
@@ -74,7 +78,10 @@ struct Binning {
     int64_t max_chunks = 0;
     int64_t slot_cells = 0;     // padded-tile cells of one adjoint slot
     uint32_t *keys = nullptr;       // M: tile of each node
+    uint32_t *sub = nullptr;        // M: window start within the tile (plane-tiled kernels)
+    uint32_t *sub2 = nullptr;       // M: the same in placement order
     uint32_t *perm = nullptr;       // M: node at each sorted position
+    uint32_t *perm2 = nullptr;      // M: placement order before the sort within tiles
     uint32_t *tile_count = nullptr; // ntiles
     uint32_t *tile_start = nullptr; // ntiles
     uint32_t *cursor = nullptr;     // ntiles
@@ -101,7 +108,25 @@ struct BinGeom {
     int logT[3];
     int64_t nt[3];
     int64_t ntiles;
+    int sub_shift;   // column cells per lane unit, log2 (FP32 1, FP64 0)
 };
+
+// Order of a node within its tile (window start lo_t relative to the tile
+// origin, tile extents T_t): plane lo_0 fastest, then the column unit
+// lo_L >> sub_shift, then the row lo_1 (d = 3).  Range: sub_range.
+__host__ __device__ inline uint32_t sub_key(int d, const uint32_t (&lo)[3], const int (&logT)[3],
+                                            int sub_shift)
+{
+    const uint32_t T0 = 1u << logT[0];
+    const uint32_t cu = lo[d - 1] >> sub_shift;
+    const uint32_t nu = (1u << logT[d - 1]) >> sub_shift;
+    return d == 3 ? lo[0] + T0 * (cu + nu * lo[1]) : lo[0] + T0 * cu;
+}
+inline uint32_t sub_range(const BinGeom &g)
+{
+    uint32_t r = (1u << g.logT[0]) * ((1u << g.logT[g.d - 1]) >> g.sub_shift);
+    return g.d == 3 ? r << g.logT[1] : r;
+}
 BinGeom bin_geom(const Binning &b);
 
 // Window constants of the plane-tiled kernels.

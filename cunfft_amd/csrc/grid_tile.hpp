@@ -130,6 +130,7 @@ constexpr LaneTab<NT> make_lanes(int G, int R, int NU, int PS, int P1, int RP, i
 // PS is the plane step between the planes one wavefront visits (the number of
 // wavefronts sharing the adjoint tile, 1 for the forward).
 template <typename T, int D, int W, int PS, int V> struct TileLayout {
+    using Real = T;
     static constexpr int PSV = PS;
     static constexpr int CB = (int)sizeof(cplx_t<T>);
     static constexpr int CPL = 16 / CB;                      // cells per lane unit
@@ -218,6 +219,10 @@ template <typename T> __device__ __forceinline__ T row_sum15(T v)
 // (the LDS executes the instructions of one wavefront in order).
 __device__ __forceinline__ void lds_order() { __asm__ volatile("" ::: "memory"); }
 
+// s_waitcnt operand waiting for LDS/scalar-memory count 0 only (vmcnt and
+// expcnt at their maxima, gfx9 encoding).
+constexpr int kWaitLgkm0 = 0xC07F;
+
 // Per-lane item of the lane table: plane q * PS, row r, unit u of a node's
 // window, as a cell offset from the window origin in the padded tile.
 struct LaneItem {
@@ -246,9 +251,17 @@ template <typename L> __device__ __forceinline__ LaneItem lane_item(int i, int l
 // grids the batch with one lane per weight, so the gridding loop evaluates no
 // exponentials: its lanes fetch their plane, row and column weights with
 // broadcast LDS reads issued together with the node's tile accesses.
-template <typename T, int D, int W, int NB> struct alignas(16) WinBatch {
+// NWP > 0 (adjoint with register runs): the plane weights of each node are
+// grouped by the plane residue a mod NWP of its NWP wavefronts, w0[n][r * OWNP + i]
+// = weight of plane r + NWP i (zero past W), so a wavefront reads its planes'
+// weights as whole 16-byte vectors.
+template <typename T, int D, int W, int NB, int NWP = 0> struct alignas(16) WinBatch {
     static constexpr int V0 = (W + 3) & ~3;
-    T w0[NB][V0];         // plane weights
+    static constexpr int OWN = NWP > 0 ? (W + NWP - 1) / NWP : 0;
+    static constexpr int VW = 16 / (int)sizeof(T);
+    static constexpr int OWNP = (OWN + VW - 1) / VW * VW;
+    static constexpr int W0N = NWP > 0 ? NWP * OWNP : V0;
+    T w0[NB][W0N];        // plane weights
     T w1[NB][W];          // row weights (d = 3)
     T w2[NB][W + 4];      // column weight of tap l at l + 1; zero at 0 and past W
     T fx[NB][D];          // c_t - m per axis (prologue input)
@@ -256,16 +269,16 @@ template <typename T, int D, int W, int NB> struct alignas(16) WinBatch {
 
 // Nodes per wavefront batch: the largest power of two up to 16 that keeps the
 // workgroups per CU the padded tile alone allows.
-template <typename T, int D, int W>
+template <typename T, int D, int W, int NWP = 0>
 constexpr int batch_nodes(int tile_bytes, int nw, int extra_per_node)
 {
     const int total = 160 * 1024;
     const int per_cu = total / tile_bytes > 0 ? total / tile_bytes : 1;
     const int budget = total / per_cu - tile_bytes;
-    return nw * ((int)sizeof(WinBatch<T, D, W, 16>) + 16 * extra_per_node) <= budget  ? 16
-           : nw * ((int)sizeof(WinBatch<T, D, W, 8>) + 8 * extra_per_node) <= budget ? 8
-           : nw * ((int)sizeof(WinBatch<T, D, W, 4>) + 4 * extra_per_node) <= budget ? 4
-                                                                                     : 2;
+    return nw * ((int)sizeof(WinBatch<T, D, W, 16, NWP>) + 16 * extra_per_node) <= budget  ? 16
+           : nw * ((int)sizeof(WinBatch<T, D, W, 8, NWP>) + 8 * extra_per_node) <= budget ? 8
+           : nw * ((int)sizeof(WinBatch<T, D, W, 4, NWP>) + 4 * extra_per_node) <= budget ? 4
+                                                                                          : 2;
 }
 
 // Window origin of a node, held by the lane that loaded it.
@@ -279,10 +292,21 @@ struct NodeRec {
 // its record and axis offsets, then every lane evaluates window weights.
 // Within one wavefront LDS accesses execute in order, so the compiler
 // barriers suffice between the phases.
-template <typename T, int D, int W, int NB, typename L>
-__device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB> &wb, int nb, int lane,
+template <typename T, int D, int W, int NB, typename L, int NWP = 0>
+__device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB, NWP> &wb, int nb, int lane,
                                                   const NodeDesc<T> &nd, const WinConst<T> &wc)
 {
+    using WB = WinBatch<T, D, W, NB, NWP>;
+    if constexpr (NWP > 0) {
+        // zero the padding of the grouped plane weights (whole array, 16-byte stores)
+        using VT = vec16_t<T>;
+        constexpr int NV = NB * WB::W0N / WB::VW;
+        VT z;
+        if constexpr (sizeof(T) == 4) z = make_float4(0.f, 0.f, 0.f, 0.f);
+        else z = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = lane; i < NV; i += 64) reinterpret_cast<VT *>(&wb.w0[0][0])[i] = z;
+    }
     NodeRec r{0, 0, 0};
     if (lane < nb) {
         const uint32_t loc = (uint32_t)nd.loc;
@@ -320,7 +344,12 @@ __device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB> &wb, int
             const T w = wc.win == WIN_KAISER_BESSEL
                             ? kb_phi_call<T>(sv[ps], (T)wc.m, wc.kbb[t])
                             : wc.nrm[t] * wexp<T>(-wc.kb[t] * sv[ps] * sv[ps]);
-            if (t == 0) wb.w0[nv[ps]][lv[ps]] = w;
+            if (t == 0) {
+                if constexpr (NWP > 0)
+                    wb.w0[nv[ps]][(lv[ps] % NWP) * WB::OWNP + lv[ps] / NWP] = w;
+                else
+                    wb.w0[nv[ps]][lv[ps]] = w;
+            }
             else if (t == D - 1) wb.w2[nv[ps]][lv[ps] + 1] = w;
             else wb.w1[nv[ps]][lv[ps]] = w;
         }
@@ -571,6 +600,53 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather_batch(const GridJob<T> 
 // read-add-write sequences stay in program order.  When a plane group can
 // reach past the window, the lanes concerned read and write a private dummy
 // unit past the tile.  fs holds the coefficients in sorted node order.
+// Padded tile (LDS) -> the chunk's slot, destination-blocked: the block that
+// lands on a neighbouring tile is contiguous.  COMP: the slot receives the
+// sum minus its Kahan term (the second tile at EOFF).
+template <typename L, bool COMP, int NT, typename VT>
+__device__ __forceinline__ void tile_to_slot(const VT *tile, cplx_t<typename L::Real> *__restrict__ slot)
+{
+    using T = typename L::Real;
+    using C = cplx_t<T>;
+    constexpr int CPL = L::CPL;
+    constexpr uint32_t EOFF = COMP ? (L::CELLS / CPL + (L::NEED_DUMMY ? 64 : 0)) * 16 : 0;
+    const int tid = threadIdx.x;
+    const C *tc = reinterpret_cast<const C *>(tile);
+#pragma unroll
+    for (int q0 = 0; q0 < L::Q0; ++q0)
+#pragma unroll
+        for (int q1 = 0; q1 < L::Q1; ++q1)
+#pragma unroll
+            for (int qL = 0; qL < L::QL; ++qL) {
+                const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1),
+                          EL = L::ext(L::TL, L::PL, qL);
+                const int n = E0 * E1 * EL;
+                C *dst = slot + L::block_off(q0, q1, qL);
+                for (int c0 = 0; c0 < n; c0 += 8 * NT) {
+                    C v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        int c = c0 + tid + u * NT;
+                        c = c < n ? c : n - 1;
+                        const int r0 = c / (E1 * EL), rr = c - r0 * (E1 * EL);
+                        const int r1 = rr / EL, rL = rr - r1 * EL;
+                        const int src = (q0 * L::T0 + r0) * L::PP + (q1 * L::T1 + r1) * L::RP +
+                                        qL * L::TL + rL;
+                        v[u] = tc[src];
+                        if constexpr (COMP) {
+                            const C e = tc[src + EOFF / sizeof(C)];
+                            v[u] = cmake<T>(v[u].x - e.x, v[u].y - e.y);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int c = c0 + tid + u * NT;
+                        if (c < n) dst[c] = v[u];
+                    }
+                }
+            }
+}
+
 template <typename T, int D, int W, int NW, int V, bool COMP>
 __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc,
                                             const cplx_t<T> *__restrict__ fs, const Chunk ch,
@@ -704,41 +780,169 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
     }
     __syncthreads();
 
-    // ---- padded tile -> slot, destination-blocked
-    const C *tc = reinterpret_cast<const C *>(tile);
+    tile_to_slot<L, COMP, NT>(tile, slot);
+}
+
+// Adjoint with the tile units held across nodes (d = 3, one plane per lane
+// group).  The nodes of a tile come sorted by row, column unit and plane of
+// their window start, so consecutive nodes often cover the same rows and
+// column units of the padded tile, and a wavefront's first plane
+// p = lo_0 + ((w - lo_0) mod NW) is then often the same as well: the units
+// the wavefront updates for the two nodes are the same.  A lane keeps them in
+// registers until a node needs other units, and only then writes them back
+// and reads the new ones, so a sequence of r such nodes costs the LDS accesses
+// of one.  The plane weights come grouped by residue (zero past the window),
+// so a node whose wavefront planes run past its window adds zero there.  Order
+// of the sums per cell: the nodes in sorted order, as in the direct update.
+template <typename T, int D, int W, int NW, int V>
+__device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__ desc,
+                                                 const cplx_t<T> *__restrict__ fs, const Chunk ch,
+                                                 cplx_t<T> *__restrict__ slot, const WinConst<T> &wc)
+{
+    using L = TileLayout<T, D, W, NW, V>;
+    static_assert(D == 3 && L::G == 1, "units held across nodes: one plane per lane group");
+    using VT = vec16_t<T>;
+    using C = cplx_t<T>;
+    constexpr int CPL = L::CPL;
+    constexpr int NT = NW * 64;
+    constexpr int NI = L::NI;
+    constexpr int OWN = L::OWN;
+    constexpr bool DUMMY = L::NEED_DUMMY;
+    constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
+    constexpr int NB = batch_nodes<T, D, W, NW>(TILE_UNITS * 16, NW, 0);
+    using WB = WinBatch<T, D, W, NB, NW>;
+    __shared__ __align__(16) VT tile[TILE_UNITS];
+    __shared__ WB wbs[NW];
+    // wave-uniform values stay in scalar registers (they drive the branch below)
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    NodeDesc<T> nd{};
+    C fv = cmake<T>(0, 0);
+    if (lane < NB && (uint32_t)lane < ch.count) {
+        nd = desc[ch.start + lane];
+        fv = fs[ch.start + lane];
+    }
+    {
+        VT z;
+        if constexpr (CPL == 2) z = make_float4(0.f, 0.f, 0.f, 0.f);
+        else z = make_double2(0.0, 0.0);
+        for (int idx = tid; idx < TILE_UNITS; idx += NT) tile[idx] = z;
+    }
+    __syncthreads();
+
+    bool ok[NI];
+    int ur[NI], uc[NI];
+    uint32_t offb[NI];
 #pragma unroll
-    for (int q0 = 0; q0 < L::Q0; ++q0)
+    for (int i = 0; i < NI; ++i) {
+        const LaneItem it = lane_item<L>(i, lane);
+        ok[i] = it.ok;
+        ur[i] = it.r;
+        uc[i] = it.u2 + 1;
+        offb[i] = (uint32_t)it.off * L::CB;
+    }
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t dummy = (uint32_t)(L::CELLS / CPL + lane) * 16;
+    WB &wb = wbs[wave];
+
+    VT v[NI][OWN];          // the held units: plane p_cur + NW i of lane item ii
+    uint32_t ad[NI][OWN];
+    int cur_rc = -1, cur_p = -1;
+    for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
+        const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
+        const NodeRec rc = batch_prologue<T, D, W, NB, L, NW>(wb, nb, lane, nd, wc);
+        const C fc = fv;
+        {
+            const uint32_t s = s0 + NB + lane;
+            if (lane < NB && s < ch.count) {
+                nd = desc[ch.start + s];
+                fv = fs[ch.start + s];
+            }
+        }
+        for (int k = 0; k < nb; ++k) {
+            const int lo0 = rlane_i(rc.lo0, k), par = rlane_i(rc.par, k);
+            const int a0 = NW == 1 ? 0 : ((wave - lo0) & (NW - 1));
+            const int p = __builtin_amdgcn_readfirstlane(lo0 + a0);
+            const int rcb = __builtin_amdgcn_readfirstlane(rlane_i(rc.nbase, k) - lo0 * L::PP);
+            const T fr = rlane_t(fc.x, k), fi = rlane_t(fc.y, k);
+            if (rcb != cur_rc || p != cur_p) {
+                // write back the held units, then read the node's (in order per wavefront)
+                if (cur_rc >= 0) {
 #pragma unroll
-        for (int q1 = 0; q1 < L::Q1; ++q1)
+                    for (int ii = 0; ii < NI; ++ii) {
+                        if (!ok[ii]) continue;
 #pragma unroll
-            for (int qL = 0; qL < L::QL; ++qL) {
-                const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1),
-                          EL = L::ext(L::TL, L::PL, qL);
-                const int n = E0 * E1 * EL;
-                C *dst = slot + L::block_off(q0, q1, qL);
-                for (int c0 = 0; c0 < n; c0 += 8 * NT) {
-                    C v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        int c = c0 + tid + u * NT;
-                        c = c < n ? c : n - 1;
-                        const int r0 = c / (E1 * EL), rr = c - r0 * (E1 * EL);
-                        const int r1 = rr / EL, rL = rr - r1 * EL;
-                        const int src = (q0 * L::T0 + r0) * L::PP + (q1 * L::T1 + r1) * L::RP +
-                                        qL * L::TL + rL;
-                        v[u] = tc[src];
-                        if constexpr (COMP) {
-                            const C e = tc[src + EOFF / sizeof(C)];
-                            v[u] = cmake<T>(v[u].x - e.x, v[u].y - e.y);
-                        }
+                        for (int i = 0; i < OWN; ++i) *reinterpret_cast<VT *>(tb + ad[ii][i]) = v[ii][i];
                     }
+                }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int c = c0 + tid + u * NT;
-                        if (c < n) dst[c] = v[u];
+                for (int ii = 0; ii < NI; ++ii) {
+                    if (!ok[ii]) continue;
+                    const uint32_t lb = (uint32_t)(p * L::PP + rcb) * L::CB + offb[ii];
+#pragma unroll
+                    for (int i = 0; i < OWN; ++i) {
+                        uint32_t a = lb + (uint32_t)(NW * i * L::PP * L::CB);
+                        if constexpr (DUMMY) a = p + NW * i < L::P0 ? a : dummy;
+                        ad[ii][i] = a;
+                        v[ii][i] = *reinterpret_cast<const VT *>(tb + a);
+                    }
+                }
+                cur_rc = rcb;
+                cur_p = p;
+            }
+            // the node's weights: its planes p + NW i (grouped by residue a0), rows, columns
+            T w0[OWN];
+            {
+                const VT *src = reinterpret_cast<const VT *>(&wb.w0[k][a0 * WB::OWNP]);
+#pragma unroll
+                for (int q = 0; q < WB::OWNP / WB::VW; ++q) {
+                    const VT e = src[q];
+                    if constexpr (sizeof(T) == 4) {
+                        if (4 * q + 0 < OWN) w0[4 * q + 0] = e.x;
+                        if (4 * q + 1 < OWN) w0[4 * q + 1] = e.y;
+                        if (4 * q + 2 < OWN) w0[4 * q + 2] = e.z;
+                        if (4 * q + 3 < OWN) w0[4 * q + 3] = e.w;
+                    } else {
+                        if (2 * q + 0 < OWN) w0[2 * q + 0] = e.x;
+                        if (2 * q + 1 < OWN) w0[2 * q + 1] = e.y;
                     }
                 }
             }
+#pragma unroll
+            for (int ii = 0; ii < NI; ++ii) {
+                const int cw = uc[ii] - par;
+                const T w1v = wb.w1[k][ur[ii]];
+                const T wu = w1v * wb.w2[k][cw];
+                const T fux = fr * wu, fuy = fi * wu;
+                T fvx = 0, fvy = 0;
+                if constexpr (CPL == 2) {
+                    const T wv = w1v * wb.w2[k][cw + 1];
+                    fvx = fr * wv;
+                    fvy = fi * wv;
+                }
+#pragma unroll
+                for (int i = 0; i < OWN; ++i) {
+                    v[ii][i].x += w0[i] * fux;
+                    v[ii][i].y += w0[i] * fuy;
+                    if constexpr (CPL == 2) {
+                        v[ii][i].z += w0[i] * fvx;
+                        v[ii][i].w += w0[i] * fvy;
+                    }
+                }
+            }
+        }
+        lds_order();
+    }
+    if (cur_rc >= 0) {
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii) {
+            if (!ok[ii]) continue;
+#pragma unroll
+            for (int i = 0; i < OWN; ++i) *reinterpret_cast<VT *>(tb + ad[ii][i]) = v[ii][i];
+        }
+    }
+    __syncthreads();
+    tile_to_slot<L, false, NT>(tile, slot);
 }
 
 template <typename T, int D, int W, int NW, int V, bool COMP>
@@ -749,8 +953,12 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
 {
     using L = TileLayout<T, D, W, NW, V>;
     if (blockIdx.x >= n_chunks[0]) return;
-    tile_spread<T, D, W, NW, V, COMP>(desc, fs, chunks[blockIdx.x],
-                                      slots + (int64_t)blockIdx.x * L::SC, wc);
+    if constexpr (D == 3 && !COMP && L::G == 1)
+        tile_spread_runs<T, D, W, NW, V>(desc, fs, chunks[blockIdx.x],
+                                         slots + (int64_t)blockIdx.x * L::SC, wc);
+    else
+        tile_spread<T, D, W, NW, V, COMP>(desc, fs, chunks[blockIdx.x],
+                                          slots + (int64_t)blockIdx.x * L::SC, wc);
 }
 
 template <typename T, int D, int W, int NW, int V>
@@ -760,8 +968,12 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread_batch(const GridJob<T> 
     using L = TileLayout<T, D, W, NW, V>;
     const BlockMap bm = map[blockIdx.x];
     const GridJob<T> &J = jobs[bm.job];
-    tile_spread<T, D, W, NW, V, false>(static_cast<const NodeDesc<T> *>(J.desc), J.fs,
-                                       J.chunks[bm.blk], J.slots + (int64_t)bm.blk * L::SC, J.wc);
+    if constexpr (D == 3 && L::G == 1)
+        tile_spread_runs<T, D, W, NW, V>(static_cast<const NodeDesc<T> *>(J.desc), J.fs,
+                                         J.chunks[bm.blk], J.slots + (int64_t)bm.blk * L::SC, J.wc);
+    else
+        tile_spread<T, D, W, NW, V, false>(static_cast<const NodeDesc<T> *>(J.desc), J.fs,
+                                           J.chunks[bm.blk], J.slots + (int64_t)bm.blk * L::SC, J.wc);
 }
 
 // Coefficients in sorted node order for the adjoint: fs[s] = f[perm[s]].
@@ -942,7 +1154,7 @@ template <typename T, int D, int W, int V> struct Waves {
     // with most lanes busy; splitting its rows over wavefronts would leave
     // most lanes idle, so the adjoint runs one wavefront on a small tile
     // (enough of them fit a CU's LDS to hide the latency)
-    static constexpr int spread = D == 2 && tile_bytes <= 16384 ? 1 : (alone ? 4 : 2);
+    static constexpr int spread = D == 2 && tile_bytes <= 16384 ? 1 : (alone || (D == 3 && sizeof(T) == 4) ? 4 : 2);
 };
 
 template <typename T> WinConst<T> win_const(const Geom &ge)

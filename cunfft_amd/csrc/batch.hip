@@ -340,10 +340,11 @@ template <typename T> struct BatchImpl {
         add_grid(0, BS_GATHER, [&](int i) { return probs[i].nch[0]; });
         // adjoint: gridding, then FFT passes 0 .. d-1
         if (planes(1)) {
-            const int slices = tile_fold_slices(d, config[1]);
             add_grid(1, BS_COEFFS, [&](int i) { return (probs[i].M + 255) / 256; });
             add_grid(1, BS_SPREAD, [&](int i) { return probs[i].nch[1]; });
-            add_grid(1, BS_FOLD, [&](int i) { return probs[i].bins[slot(1)].ntiles * slices; });
+            add_grid(1, BS_FOLD, [&](int i) {
+                return tile_fold_blocks(d, config[1], probs[i].bins[slot(1)].ntiles, (int)sizeof(T));
+            });
         } else {
             add_grid(1, BS_ZERO, [&](int i) { return (probs[i].n_total + 1023) / 1024; });
             add_grid(1, BS_SCATTER_LANES, [&](int i) { return probs[i].nch[1]; });

@@ -172,7 +172,7 @@ enum BatchStage : int {
     BS_COUNT
 };
 // Blocks of job-independent size in the fold: slices of 512 core cells.
-int tile_fold_slices(int d, int variant);
+int64_t tile_fold_blocks(int d, int variant, int64_t ntiles, int elem_bytes);   // fold workgroups of a job
 // One launch of stage st over the jobs of a batch whose problems share the
 // dimension d, window width W, precision T and kernel configuration
 // (plane-tiled variant 0..kTileVariants-1, or kTileVariants for node lanes).

@@ -153,7 +153,8 @@ template <typename T, int D, int W, int PS, int V> struct TileLayout {
     static constexpr int RP = choose_pitch(G, R, NU, PS, P0, P1, RL, CB, NG);
     static constexpr int PP = P1 * RP;
     static constexpr int CELLS = P0 * PP;
-    static constexpr int SC = P0 * P1 * PL;                  // adjoint slot cells
+    // adjoint slot cells, rounded up to whole 16-byte units so every slot starts aligned
+    static constexpr int SC = (P0 * P1 * PL + CPL - 1) / CPL * CPL;
     static constexpr int Q0 = (P0 + T0 - 1) / T0, Q1 = (P1 + T1 - 1) / T1,
                          QL = (PL + TL - 1) / TL;            // slot blocks per axis
     static constexpr LaneTab<NI * 64> lanes = make_lanes<NI * 64>(G, R, NU, PS, P1, RP, CB, NG);
@@ -1116,14 +1117,176 @@ __device__ __forceinline__ void tile_fold(const cplx_t<T> *__restrict__ slots,
         }
 }
 
+// Two consecutive cells with one 16-byte load (FP32; the FP64 form is two loads).
+__device__ __forceinline__ void load_pair(const float2 *p, float2 &a, float2 &b)
+{
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    a = make_float2(t.x, t.y);
+    b = make_float2(t.z, t.w);
+}
+__device__ __forceinline__ void load_pair(const double2 *p, double2 &a, double2 &b)
+{
+    a = p[0];
+    b = p[1];
+}
+__device__ __forceinline__ void store_pair(float2 *p, float2 a, float2 b)
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(a.x, a.y, b.x, b.y);
+}
+__device__ __forceinline__ void store_pair(double2 *p, double2 a, double2 b)
+{
+    p[0] = a;
+    p[1] = b;
+}
+
+// 3D fold by column segments: a thread of a tile owns the SEGC = 64 / (cell
+// bytes) core cells (r0, r1, seg SEGC .. seg SEGC + SEGC-1) and sums them over
+// all candidate slot blocks.  In a slot block of extents (E0, E1, EL) those
+// cells are consecutive, so a thread reads 64 contiguous bytes with 16-byte
+// loads and consecutive threads read consecutive 64-byte pieces of the block.
+// A block that does not cover the thread's column (E_t = 1 on a halo block:
+// only r_t = 0 is covered) is read at r_t = 0 and not added.  Candidates go
+// by (q0, q1): the first chunks of the q_L blocks in flight together, then the
+// further chunks of dense tiles.  The order of the sums is fixed, so the grid
+// is reproducible for a given binning.
+// The column-segment fold serves FP32 3D grids (0.51 -> 0.44 ms at 256^3); for
+// FP64 the per-cell fold (tile_fold), whose loads are already 16 bytes per lane and
+// contiguous over the wavefront, measured faster (0.74 vs 0.78 ms).
+template <typename T, int D> constexpr bool kFoldColumns = D == 3 && sizeof(T) == 4;
+
+template <typename L> struct Fold3Shape {
+    static constexpr int SEGC = 64 / L::CB;                 // cells per thread
+    static constexpr int S = L::TL / SEGC;                  // segments per column
+    static constexpr int TPT = L::T0 * L::T1 * S;           // threads per tile
+    static constexpr int TPW = TPT <= 256 ? 256 / TPT : 1;  // tiles per workgroup
+    static constexpr int WPT = TPT > 256 ? TPT / 256 : 1;   // workgroups per tile
+    static_assert(L::TL % SEGC == 0 && TPT % 64 == 0 && (256 % TPT == 0 || TPT % 256 == 0),
+                  "column segments tile the core and whole wavefronts");
+};
+
+template <typename T, int W, int PS, int V, bool COMP>
+__device__ __forceinline__ void tile_fold3(const cplx_t<T> *__restrict__ slots,
+                                           const uint32_t *__restrict__ tile_count,
+                                           const uint32_t *__restrict__ chunk_start,
+                                           cplx_t<T> *__restrict__ g, const Geom &ge,
+                                           const BinGeom &bg, uint32_t blk)
+{
+    using L = TileLayout<T, 3, W, PS, V>;
+    using F = Fold3Shape<L>;
+    using C = cplx_t<T>;
+    constexpr int TL = L::TL, SEGC = F::SEGC, S = F::S;
+    constexpr int ELAST = L::PL - (L::QL - 1) * TL;          // extent of the last q_L block
+    constexpr int NLAST = ELAST < SEGC ? ELAST : SEGC;       // cells a thread loads there
+    static_assert(L::T0 % 2 == 0 && L::T1 % 2 == 0 && TL % 2 == 0,
+                  "even tile extents keep slot rows and grid rows 16-byte aligned");
+    const int tid = threadIdx.x;
+    const uint32_t tile = F::TPT <= 256 ? blk * F::TPW + (uint32_t)(tid / F::TPT) : blk / F::WPT;
+    if (tile >= (uint32_t)bg.ntiles) return;   // whole wavefronts
+    const int tt = F::TPT <= 256 ? tid % F::TPT : (int)(blk % F::WPT) * 256 + tid;
+    const int col = tt / S, seg = tt % S;
+    const int r0 = col / L::T1, r1 = col % L::T1;
+    uint32_t tk[3];
+    {
+        uint32_t rem = tile;
+        for (int t = 2; t >= 0; --t) {
+            tk[t] = rem % (uint32_t)bg.nt[t];
+            rem /= (uint32_t)bg.nt[t];
+        }
+    }
+    T sx[SEGC], sy[SEGC], cx[SEGC], cy[SEGC];
+#pragma unroll
+    for (int i = 0; i < SEGC; ++i) sx[i] = sy[i] = cx[i] = cy[i] = 0;
+    // the thread's cells of block (q0, q1, qL) of slot sp; full blocks: SEGC
+    // aligned cells, the last q_L block: NLAST cells (index clamped in the block)
+    auto load_full = [&](const C *sp, int o, C *dst) {
+#pragma unroll
+        for (int i = 0; i < SEGC / 2; ++i) load_pair(sp + o + 2 * i, dst[2 * i], dst[2 * i + 1]);
+    };
+    auto load_last = [&](const C *sp, int o, C *dst) {
+#pragma unroll
+        for (int i = 0; i < NLAST; ++i) {
+            const int rl = seg * SEGC + i;
+            dst[i] = sp[o + (rl < ELAST ? rl : ELAST - 1)];
+        }
+    };
+#pragma unroll
+    for (int q0 = 0; q0 < L::Q0; ++q0)
+#pragma unroll
+        for (int q1 = 0; q1 < L::Q1; ++q1) {
+            constexpr int QL = L::QL;
+            const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1);
+            const bool in01 = r0 < E0 && r1 < E1;
+            const int rc = ((r0 < E0 ? r0 : 0) * E1 + (r1 < E1 ? r1 : 0));
+            uint32_t c0[QL], nch[QL];
+            bool ok[QL];
+            C v[QL][SEGC];
+#pragma unroll
+            for (int qL = 0; qL < QL; ++qL) {
+                const uint32_t n0 = (uint32_t)bg.nt[0], n1 = (uint32_t)bg.nt[1], n2 = (uint32_t)bg.nt[2];
+                const uint32_t src = ((tk[0] + 4 * n0 - q0) % n0 * n1 + (tk[1] + 4 * n1 - q1) % n1) * n2 +
+                                     (tk[2] + 4 * n2 - qL) % n2;
+                const uint32_t cnt = tile_count[src];
+                c0[qL] = chunk_start[src];
+                nch[qL] = (cnt + kChunk - 1) / kChunk;
+                ok[qL] = cnt > 0 && in01;
+            }
+#pragma unroll
+            for (int qL = 0; qL < QL; ++qL) {
+                // a candidate with no nodes reads chunk 0 and adds nothing
+                const C *sp = slots + (int64_t)(nch[qL] > 0 ? c0[qL] : 0) * L::SC;
+                const int EL = L::ext(TL, L::PL, qL);
+                const int o = L::block_off(q0, q1, qL) + rc * EL;
+                if (EL == TL) load_full(sp, o + seg * SEGC, v[qL]);
+                else load_last(sp, o, v[qL]);
+            }
+#pragma unroll
+            for (int qL = 0; qL < QL; ++qL) {
+                const int EL = L::ext(TL, L::PL, qL);
+#pragma unroll
+                for (int i = 0; i < SEGC; ++i)
+                    if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && ok[qL]) {
+                        fold_add<COMP>(sx[i], cx[i], v[qL][i].x);
+                        fold_add<COMP>(sy[i], cy[i], v[qL][i].y);
+                    }
+            }
+            // further chunks of dense candidate tiles
+#pragma unroll
+            for (int qL = 0; qL < QL; ++qL) {
+                const int EL = L::ext(TL, L::PL, qL);
+                const int o = L::block_off(q0, q1, qL) + rc * EL;
+                for (uint32_t c = 1; c < nch[qL]; ++c) {
+                    const C *sp = slots + (int64_t)(c0[qL] + c) * L::SC;
+                    C w[SEGC];
+                    if (EL == TL) load_full(sp, o + seg * SEGC, w);
+                    else load_last(sp, o, w);
+#pragma unroll
+                    for (int i = 0; i < SEGC; ++i)
+                        if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && in01) {
+                            fold_add<COMP>(sx[i], cx[i], w[i].x);
+                            fold_add<COMP>(sy[i], cy[i], w[i].y);
+                        }
+                }
+            }
+        }
+    const uint32_t n1 = (uint32_t)ge.n[1], n2 = (uint32_t)ge.n[2];
+    C *dst = g + (size_t)((tk[0] * L::T0 + r0) * n1 + tk[1] * L::T1 + r1) * n2 + tk[2] * TL + seg * SEGC;
+#pragma unroll
+    for (int i = 0; i < SEGC / 2; ++i)
+        store_pair(dst + 2 * i, cmake<T>(sx[2 * i] + cx[2 * i], sy[2 * i] + cy[2 * i]),
+                   cmake<T>(sx[2 * i + 1] + cx[2 * i + 1], sy[2 * i + 1] + cy[2 * i + 1]));
+}
+
 template <typename T, int D, int W, int PS, int V, bool COMP>
 __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__ slots,
                                                    const uint32_t *__restrict__ tile_count,
                                                    const uint32_t *__restrict__ chunk_start,
                                                    cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
 {
-    tile_fold<T, D, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x,
-                                    blockIdx.y);
+    if constexpr (kFoldColumns<T, D>)
+        tile_fold3<T, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x);
+    else
+        tile_fold<T, D, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x,
+                                        blockIdx.y);
 }
 
 // Batched fold: block blk of a job is tile blk / slices, slice blk % slices.
@@ -1136,8 +1299,11 @@ __global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__res
     constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
     const BlockMap bm = map[blockIdx.x];
     const GridJob<T> &J = jobs[bm.job];
-    tile_fold<T, D, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg,
-                                     bm.blk / SLICES, (int)(bm.blk % SLICES));
+    if constexpr (kFoldColumns<T, D>)
+        tile_fold3<T, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg, bm.blk);
+    else
+        tile_fold<T, D, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg,
+                                         bm.blk / SLICES, (int)(bm.blk % SLICES));
 }
 
 // ------------------------------------------------------------ dispatch --
@@ -1282,7 +1448,9 @@ template <typename T> struct SpreadOp {
             using L = TileLayout<T, D, W, NWS, V>;
             constexpr int CORE = L::T0 * L::T1 * L::TL;
             constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
-            const dim3 fold_grid((unsigned)b.ntiles, SLICES);
+            const dim3 fold_grid = kFoldColumns<T, D>
+                                       ? dim3((unsigned)tile_fold_blocks(3, V, b.ntiles, sizeof(T)), 1)
+                                       : dim3((unsigned)b.ntiles, SLICES);
             if constexpr (fits_comp<T, D, W, V>()) {
                 if (b.compensated) {
                     k_tile_spread<T, D, W, NWS, V, true><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(

@@ -270,13 +270,15 @@ template <typename T, int D, int W, int NB, int NWP = 0> struct alignas(16) WinB
 
 // Nodes per wavefront batch: the largest power of two up to 16 that keeps the
 // workgroups per CU the padded tile alone allows.
-template <typename T, int D, int W, int NWP = 0>
+template <typename T, int D, int W, int NWP = 0, int MAXNB = 16>
 constexpr int batch_nodes(int tile_bytes, int nw, int extra_per_node)
 {
     const int total = 160 * 1024;
     const int per_cu = total / tile_bytes > 0 ? total / tile_bytes : 1;
     const int budget = total / per_cu - tile_bytes;
-    return nw * ((int)sizeof(WinBatch<T, D, W, 16, NWP>) + 16 * extra_per_node) <= budget  ? 16
+    return MAXNB >= 32 && nw * ((int)sizeof(WinBatch<T, D, W, 32, NWP>) + 32 * extra_per_node) <= budget
+               ? 32
+           : nw * ((int)sizeof(WinBatch<T, D, W, 16, NWP>) + 16 * extra_per_node) <= budget  ? 16
            : nw * ((int)sizeof(WinBatch<T, D, W, 8, NWP>) + 8 * extra_per_node) <= budget ? 8
            : nw * ((int)sizeof(WinBatch<T, D, W, 4, NWP>) + 4 * extra_per_node) <= budget ? 4
                                                                                           : 2;
@@ -356,6 +358,74 @@ __device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB, NWP> &wb
         }
     }
     lds_order();
+    return r;
+}
+
+// Batch prologue shared by the NW wavefronts of a workgroup that all walk
+// the same nodes (the 3D adjoint): each wavefront turns its lane n < nb
+// descriptor into the node's record (as batch_prologue), wavefront 0 writes
+// the per-node inputs, and the D axes x ceil(nb W / 64) weight passes are
+// dealt over the wavefronts, so a weight is evaluated once per workgroup, not
+// once per wavefront.  Ends with a barrier; the caller must not rewrite wb
+// before every wavefront has read the previous batch (a barrier).
+template <typename T, int D, int W, int NB, typename L, int NWP, int NW>
+__device__ __forceinline__ NodeRec batch_prologue_wg(WinBatch<T, D, W, NB, NWP> &wb, int nb, int lane,
+                                                     int wave, const NodeDesc<T> &nd,
+                                                     const WinConst<T> &wc)
+{
+    using WB = WinBatch<T, D, W, NB, NWP>;
+    NodeRec r{0, 0, 0};
+    if (lane < nb) {
+        const uint32_t loc = (uint32_t)nd.loc;
+        r.lo0 = loc & 255;
+        const int lo1 = D == 3 ? (loc >> 8) & 255 : 0;
+        const int loL = (loc >> (8 * (D - 1))) & 255;
+        r.par = L::CPL == 2 ? (loL & 1) : 0;
+        r.nbase = r.lo0 * L::PP + lo1 * L::RP + loL - r.par;
+    }
+    if (wave == 0) {
+        if constexpr (NWP > 0) {
+            using VT = vec16_t<T>;
+            constexpr int NV = NB * WB::W0N / WB::VW;
+            VT z;
+            if constexpr (sizeof(T) == 4) z = make_float4(0.f, 0.f, 0.f, 0.f);
+            else z = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = lane; i < NV; i += 64) reinterpret_cast<VT *>(&wb.w0[0][0])[i] = z;
+        }
+        if (lane < nb) {
+#pragma unroll
+            for (int t = 0; t < D; ++t) wb.fx[lane][t] = nd.f[t];
+            wb.w2[lane][0] = (T)0;
+            wb.w2[lane][W + 1] = (T)0;
+            wb.w2[lane][W + 2] = (T)0;
+            wb.w2[lane][W + 3] = (T)0;
+        }
+    }
+    __syncthreads();
+    constexpr int NPASS = (NB * W + 63) / 64;
+    for (int cmb = wave; cmb < D * NPASS; cmb += NW) {   // wavefront-uniform axis and pass
+        const int t = cmb / NPASS, ps = cmb - t * NPASS;
+        const int it = lane + 64 * ps;
+        if (it < nb * W) {
+            const int n = it / W, l = it - n * W;
+            const T sv = wb.fx[n][t] + (T)(wc.m - l);
+            const T kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
+            const T nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
+            const T kbbt = t == 0 ? wc.kbb[0] : (t == 1 ? wc.kbb[1] : wc.kbb[2]);
+            const T w = wc.win == WIN_KAISER_BESSEL ? kb_phi_call<T>(sv, (T)wc.m, kbbt)
+                                                    : nrt * wexp<T>(-kbt * sv * sv);
+            if (t == 0) {
+                if constexpr (NWP > 0) wb.w0[n][(l % NWP) * WB::OWNP + l / NWP] = w;
+                else wb.w0[n][l] = w;
+            } else if (t == D - 1) {
+                wb.w2[n][l + 1] = w;
+            } else {
+                wb.w1[n][l] = w;
+            }
+        }
+    }
+    __syncthreads();
     return r;
 }
 
@@ -810,10 +880,10 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
     constexpr int OWN = L::OWN;
     constexpr bool DUMMY = L::NEED_DUMMY;
     constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
-    constexpr int NB = batch_nodes<T, D, W, NW>(TILE_UNITS * 16, NW, 0);
+    constexpr int NB = batch_nodes<T, D, W, NW, 32>(TILE_UNITS * 16, 1, 0);
     using WB = WinBatch<T, D, W, NB, NW>;
     __shared__ __align__(16) VT tile[TILE_UNITS];
-    __shared__ WB wbs[NW];
+    __shared__ WB wb;   // one batch of weights for all wavefronts
     // wave-uniform values stay in scalar registers (they drive the branch below)
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
@@ -844,14 +914,14 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
     }
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t dummy = (uint32_t)(L::CELLS / CPL + lane) * 16;
-    WB &wb = wbs[wave];
 
     VT v[NI][OWN];          // the held units: plane p_cur + NW i of lane item ii
     uint32_t ad[NI][OWN];
     int cur_rc = -1, cur_p = -1;
     for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
         const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
-        const NodeRec rc = batch_prologue<T, D, W, NB, L, NW>(wb, nb, lane, nd, wc);
+        if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch
+        const NodeRec rc = batch_prologue_wg<T, D, W, NB, L, NW, NW>(wb, nb, lane, wave, nd, wc);
         const C fc = fv;
         {
             const uint32_t s = s0 + NB + lane;

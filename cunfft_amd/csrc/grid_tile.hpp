@@ -1378,10 +1378,13 @@ __global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__res
 
 // ------------------------------------------------------------ dispatch --
 // Wavefronts per workgroup.  A tile small enough for two or more workgroups
-// per CU gets 4 forward wavefronts (node batches) and 2 adjoint wavefronts
-// (each owning every other plane); measured on the FP32 8^3 tile: faster than
-// 1 and 4 adjoint wavefronts.  A tile that fills the CU's LDS alone gets 8 and
-// 4, so the CU still holds enough wavefronts to cover the LDS latency.
+// per CU gets 4 forward wavefronts (node batches); a tile that fills the CU's
+// LDS alone gets 8, so the CU still holds enough wavefronts to cover the LDS
+// latency.  The 3D adjoint runs 4 wavefronts (each owning every fourth
+// plane): with the units held across nodes and the shared weight batches it
+// is bound by instruction issue at 3 workgroups per CU, and the extra
+// wavefronts beat the two dummy planes they add (FP32 1.33 -> 1.30 ms, FP64
+// 3.04 -> 2.81 ms at 256^3 against 2 wavefronts).
 template <typename T, int D, int W, int V> struct Waves {
     static constexpr int tile_bytes = TileLayout<T, D, W, 1, V>::CELLS * TileLayout<T, D, W, 1, V>::CB;
     static constexpr bool alone = 2 * (tile_bytes + 1024) > 160 * 1024;
@@ -1389,8 +1392,8 @@ template <typename T, int D, int W, int V> struct Waves {
     // 2D: a node's window is W rows of W cells, which one wavefront covers
     // with most lanes busy; splitting its rows over wavefronts would leave
     // most lanes idle, so the adjoint runs one wavefront on a small tile
-    // (enough of them fit a CU's LDS to hide the latency)
-    static constexpr int spread = D == 2 && tile_bytes <= 16384 ? 1 : (alone || (D == 3 && sizeof(T) == 4) ? 4 : 2);
+    // (enough of them fit a CU's LDS to hide the latency), else 2
+    static constexpr int spread = D == 2 && tile_bytes <= 16384 ? 1 : (alone || D == 3 ? 4 : 2);
 };
 
 template <typename T> WinConst<T> win_const(const Geom &ge)

@@ -516,29 +516,35 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
         const uint32_t s0 = D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : (uint32_t)ge.n[1];
         VT *tv = reinterpret_cast<VT *>(tile);
         const VT *gv = reinterpret_cast<const VT *>(g);
+        // A thread keeps one unit column un of the rows; its RS = NT / NR
+        // sibling threads per column take rows r0, r0 + RS, ... (NT mod NR
+        // threads idle), so the column's wrapped grid offset is computed once.
         // 8 loads in flight per thread before the LDS stores (one at a time,
-        // each would wait out the full memory latency)
-        constexpr int TOT = L::P0 * L::P1 * NR, SB = 8;
-        for (int i0 = tid; i0 < TOT; i0 += SB * NT) {
-            VT v[SB];
-            int at[SB];
+        // each would wait out the full memory latency).
+        constexpr int ROWS = L::P0 * L::P1, RS = NT / NR, SB = 8;
+        static_assert(RS >= 1, "a workgroup covers a padded row");
+        const int un = tid % NR, r0 = tid / NR;
+        // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
+        const uint32_t colc = wrap_hi32(o[D - 1] + CPL * un, nL);
+        if (r0 < RS) {
+            for (int rb = r0; rb < ROWS; rb += SB * RS) {
+                VT v[SB];
+                int at[SB];
 #pragma unroll
-            for (int u = 0; u < SB; ++u) {
-                const int idx = i0 + u * NT < TOT ? i0 + u * NT : TOT - 1;
-                const int row = idx / NR, un = idx - row * NR;
-                const int p0 = row / L::P1, p1 = row - p0 * L::P1;
-                // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
-                uint32_t cell =
-                    wrap_hi32(o[0] + p0, n0) * s0 + wrap_hi32(o[D - 1] + CPL * un, nL);
-                if (D == 3) cell += wrap_hi32(o[1] + p1, n1) * s1;
-                at[u] = (row * L::RP) / CPL + un;
-                v[u] = gv[cell / CPL];
+                for (int u = 0; u < SB; ++u) {
+                    const int row = rb + u * RS < ROWS ? rb + u * RS : ROWS - 1;
+                    const int p0 = row / L::P1, p1 = row - p0 * L::P1;
+                    uint32_t cell = wrap_hi32(o[0] + p0, n0) * s0 + colc;
+                    if (D == 3) cell += wrap_hi32(o[1] + p1, n1) * s1;
+                    at[u] = (row * L::RP) / CPL + un;
+                    v[u] = gv[cell / CPL];
+                }
+                // unconditional stores (a thread past the last row rewrites it
+                // with its own value): a conditional store would pull its load
+                // back into the branch, one round trip at a time
+#pragma unroll
+                for (int u = 0; u < SB; ++u) tv[at[u]] = v[u];
             }
-            // unconditional stores (a lane past the end rewrites the last unit
-            // with its own value): a conditional store would pull its load
-            // back into the branch, one round trip at a time
-#pragma unroll
-            for (int u = 0; u < SB; ++u) tv[at[u]] = v[u];
         }
     }
     __syncthreads();

@@ -923,7 +923,8 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
 
     VT v[NI][OWN];          // the held units: plane p_cur + NW i of lane item ii
     uint32_t ad[NI][OWN];
-    int cur_rc = -1, cur_p = -1;
+    int cur_key = -1;
+    static_assert(L::T0 + NW - 2 < 64, "a wavefront's first plane fits the key's low 6 bits");
     for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
         const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
         if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch
@@ -936,15 +937,27 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
                 fv = fs[ch.start + s];
             }
         }
+        // lane n's view of node n for this wavefront: key (row/column base and
+        // first plane p), tile byte base of plane p, and element offsets of its
+        // plane weights (residue a0) and column weights (parity shift)
+        int nkey, ntb, nw0, nw2;
+        {
+            const int a0 = NW == 1 ? 0 : ((wave - rc.lo0) & (NW - 1));
+            const int p = rc.lo0 + a0;
+            const int rcb = rc.nbase - rc.lo0 * L::PP;
+            nkey = rcb * 64 + p;
+            ntb = (p * L::PP + rcb) * L::CB;
+            nw0 = lane * WB::W0N + a0 * WB::OWNP;
+            nw2 = lane * (W + 4) - rc.par;
+        }
+        const T *w0base = &wb.w0[0][0];
+        const T *w2base = &wb.w2[0][0];
         for (int k = 0; k < nb; ++k) {
-            const int lo0 = rlane_i(rc.lo0, k), par = rlane_i(rc.par, k);
-            const int a0 = NW == 1 ? 0 : ((wave - lo0) & (NW - 1));
-            const int p = __builtin_amdgcn_readfirstlane(lo0 + a0);
-            const int rcb = __builtin_amdgcn_readfirstlane(rlane_i(rc.nbase, k) - lo0 * L::PP);
+            const int key = __builtin_amdgcn_readfirstlane(rlane_i(nkey, k));
             const T fr = rlane_t(fc.x, k), fi = rlane_t(fc.y, k);
-            if (rcb != cur_rc || p != cur_p) {
+            if (key != cur_key) {
                 // write back the held units, then read the node's (in order per wavefront)
-                if (cur_rc >= 0) {
+                if (cur_key >= 0) {
 #pragma unroll
                     for (int ii = 0; ii < NI; ++ii) {
                         if (!ok[ii]) continue;
@@ -952,10 +965,12 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
                         for (int i = 0; i < OWN; ++i) *reinterpret_cast<VT *>(tb + ad[ii][i]) = v[ii][i];
                     }
                 }
+                const int p = key & 63;
+                const uint32_t tbase = (uint32_t)rlane_i(ntb, k);
 #pragma unroll
                 for (int ii = 0; ii < NI; ++ii) {
                     if (!ok[ii]) continue;
-                    const uint32_t lb = (uint32_t)(p * L::PP + rcb) * L::CB + offb[ii];
+                    const uint32_t lb = tbase + offb[ii];
 #pragma unroll
                     for (int i = 0; i < OWN; ++i) {
                         uint32_t a = lb + (uint32_t)(NW * i * L::PP * L::CB);
@@ -964,13 +979,12 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
                         v[ii][i] = *reinterpret_cast<const VT *>(tb + a);
                     }
                 }
-                cur_rc = rcb;
-                cur_p = p;
+                cur_key = key;
             }
             // the node's weights: its planes p + NW i (grouped by residue a0), rows, columns
             T w0[OWN];
             {
-                const VT *src = reinterpret_cast<const VT *>(&wb.w0[k][a0 * WB::OWNP]);
+                const VT *src = reinterpret_cast<const VT *>(w0base + rlane_i(nw0, k));
 #pragma unroll
                 for (int q = 0; q < WB::OWNP / WB::VW; ++q) {
                     const VT e = src[q];
@@ -985,15 +999,15 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
                     }
                 }
             }
+            const T *w2k = w2base + rlane_i(nw2, k);
 #pragma unroll
             for (int ii = 0; ii < NI; ++ii) {
-                const int cw = uc[ii] - par;
                 const T w1v = wb.w1[k][ur[ii]];
-                const T wu = w1v * wb.w2[k][cw];
+                const T wu = w1v * w2k[uc[ii]];
                 const T fux = fr * wu, fuy = fi * wu;
                 T fvx = 0, fvy = 0;
                 if constexpr (CPL == 2) {
-                    const T wv = w1v * wb.w2[k][cw + 1];
+                    const T wv = w1v * w2k[uc[ii] + 1];
                     fvx = fr * wv;
                     fvy = fi * wv;
                 }
@@ -1010,7 +1024,7 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
         }
         lds_order();
     }
-    if (cur_rc >= 0) {
+    if (cur_key >= 0) {
 #pragma unroll
         for (int ii = 0; ii < NI; ++ii) {
             if (!ok[ii]) continue;

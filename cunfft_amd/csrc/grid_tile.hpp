@@ -776,16 +776,25 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
                 fv = fs[ch.start + s];
             }
         }
+        // lane n's view of node n for this wavefront, computed once per batch:
+        // first plane residue a0, tile byte base, column weight offset
+        int na0, ntb, nw2;
+        {
+            na0 = NW == 1 ? 0 : ((wave - rc.lo0) & (NW - 1));
+            ntb = (rc.nbase + na0 * L::PP) * L::CB;
+            nw2 = lane * (W + 4) - rc.par + 1;
+        }
+        const T *w2base = &wb.w2[0][0];
         for (int k = 0; k < nb; ++k) {
-            const int lo0 = rlane_i(rc.lo0, k), par = rlane_i(rc.par, k);
-            const int a0 = NW == 1 ? 0 : ((wave - lo0) & (NW - 1));
-            const int nbase = rlane_i(rc.nbase, k) + a0 * L::PP;
+            const int a0 = NW == 1 ? 0 : rlane_i(na0, k);
+            const uint32_t tbase = (uint32_t)rlane_i(ntb, k);
+            const T *w2k = w2base + rlane_i(nw2, k);
             const T fr = rlane_t(fc.x, k), fi = rlane_t(fc.y, k);
 #pragma unroll
             for (int i = 0; i < L::NI; ++i) {
                 if (!it[i].ok) continue;
-                const uint32_t lb = (uint32_t)(nbase + it[i].off) * L::CB;
-                const int cw = it[i].u2 - par + 1;
+                const uint32_t lb = tbase + (uint32_t)it[i].off * L::CB;
+                const int cw = it[i].u2;
 #pragma unroll
                 for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
                     VT v[PB], ev[PB];
@@ -808,8 +817,8 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
                     // issued, and all of them before any use: one LDS round trip
                     lds_order();
                     const T w1v = D == 3 ? wb.w1[k][it[i].r] : (T)1;
-                    const T w2a = wb.w2[k][cw];
-                    const T w2b = CPL == 2 ? wb.w2[k][cw + 1] : (T)0;
+                    const T w2a = w2k[cw];
+                    const T w2b = CPL == 2 ? w2k[cw + 1] : (T)0;
                     T w0r[PB];
 #pragma unroll
                     for (int j = 0; j < PB; ++j) {

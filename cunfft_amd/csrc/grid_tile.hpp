@@ -161,6 +161,15 @@ template <typename T, int D, int W, int PS, int V> struct TileLayout {
     static constexpr bool fits = (long)CELLS * CB <= kLdsMax && W <= 18 && NU <= 31 && G < 64;
     // a plane group of some node can reach past its window (adjoint dummy units needed)
     static constexpr bool NEED_DUMMY = (PS - 1) + PS * (NGRP * G - 1) >= W;
+    // A lane's dummy unit: the spare 16 bytes at the end of padded row `lane`
+    // when the pitch leaves them (no row access reaches past RL cells), else
+    // one of 64 units appended to the tile.
+    static constexpr bool SPARE = (RP - RL) * CB >= 16 && (RL * CB) % 16 == 0 && P0 * P1 >= 64;
+    static constexpr int DUMMY_UNITS = NEED_DUMMY && !SPARE ? 64 : 0;
+    __device__ static uint32_t dummy_byte(int lane)
+    {
+        return SPARE ? (uint32_t)(lane * RP + RL) * CB : (uint32_t)(CELLS / CPL + lane) * 16;
+    }
     // planes loaded per batch: at most 12 outstanding 16-byte LDS reads
     static constexpr int PB = 12 / NI > 0 ? (12 / NI < NGRP ? 12 / NI : NGRP) : 1;
 
@@ -263,19 +272,24 @@ template <typename T, int D, int W, int NB, int NWP = 0> struct alignas(16) WinB
     static constexpr int OWNP = (OWN + VW - 1) / VW * VW;
     static constexpr int W0N = NWP > 0 ? NWP * OWNP : V0;
     T w0[NB][W0N];        // plane weights
-    T w1[NB][W];          // row weights (d = 3)
+    // row weights (d = 3); NWP > 0: times the node's coefficient, (re, im) per tap
+    T w1[NB][NWP > 0 ? 2 * W : W];
     T w2[NB][W + 4];      // column weight of tap l at l + 1; zero at 0 and past W
     T fx[NB][D];          // c_t - m per axis (prologue input)
 };
 
-// Nodes per wavefront batch: the largest power of two up to 16 that keeps the
-// workgroups per CU the padded tile alone allows.
+// Nodes per wavefront batch: the largest power of two up to MAXNB that keeps
+// the workgroups per CU the padded tile alone allows.  LDS is handed out in
+// granules (a 54048-byte adjoint workgroup ran two per CU where a 52768-byte
+// one ran three), so the budget is rounded down to kLdsGranule.
+constexpr int kLdsGranule = 2048;   // conservative LDS allocation granule (bytes)
+
 template <typename T, int D, int W, int NWP = 0, int MAXNB = 16>
 constexpr int batch_nodes(int tile_bytes, int nw, int extra_per_node)
 {
     const int total = 160 * 1024;
     const int per_cu = total / tile_bytes > 0 ? total / tile_bytes : 1;
-    const int budget = total / per_cu - tile_bytes;
+    const int budget = total / per_cu / kLdsGranule * kLdsGranule - tile_bytes;
     return MAXNB >= 32 && nw * ((int)sizeof(WinBatch<T, D, W, 32, NWP>) + 32 * extra_per_node) <= budget
                ? 32
            : nw * ((int)sizeof(WinBatch<T, D, W, 16, NWP>) + 16 * extra_per_node) <= budget  ? 16
@@ -368,11 +382,25 @@ __device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB, NWP> &wb
 // dealt over the wavefronts, so a weight is evaluated once per workgroup, not
 // once per wavefront.  Ends with a barrier; the caller must not rewrite wb
 // before every wavefront has read the previous batch (a barrier).
+template <typename T> __device__ __forceinline__ T lane_read(T v, int src);
+template <> __device__ __forceinline__ float lane_read<float>(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(v)));
+}
+template <> __device__ __forceinline__ double lane_read<double>(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src * 4, (int)(uint32_t)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(src * 4, (int)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
 template <typename T, int D, int W, int NB, typename L, int NWP, int NW>
 __device__ __forceinline__ NodeRec batch_prologue_wg(WinBatch<T, D, W, NB, NWP> &wb, int nb, int lane,
                                                      int wave, const NodeDesc<T> &nd,
-                                                     const WinConst<T> &wc)
+                                                     const cplx_t<T> &fcn, const WinConst<T> &wc)
 {
+    static_assert(D == 3 && NWP > 0, "row weights are stored times the coefficient");
     using WB = WinBatch<T, D, W, NB, NWP>;
     NodeRec r{0, 0, 0};
     if (lane < nb) {
@@ -407,6 +435,10 @@ __device__ __forceinline__ NodeRec batch_prologue_wg(WinBatch<T, D, W, NB, NWP> 
     for (int cmb = wave; cmb < D * NPASS; cmb += NW) {   // wavefront-uniform axis and pass
         const int t = cmb / NPASS, ps = cmb - t * NPASS;
         const int it = lane + 64 * ps;
+        // the row pass needs node n's coefficient, held by lane n of every wavefront
+        const int nn = it / W < NB ? it / W : NB - 1;
+        const T fr = t == 1 ? lane_read<T>(fcn.x, nn) : (T)0;
+        const T fi = t == 1 ? lane_read<T>(fcn.y, nn) : (T)0;
         if (it < nb * W) {
             const int n = it / W, l = it - n * W;
             const T sv = wb.fx[n][t] + (T)(wc.m - l);
@@ -421,7 +453,8 @@ __device__ __forceinline__ NodeRec batch_prologue_wg(WinBatch<T, D, W, NB, NWP> 
             } else if (t == D - 1) {
                 wb.w2[n][l + 1] = w;
             } else {
-                wb.w1[n][l] = w;
+                wb.w1[n][2 * l] = w * fr;
+                wb.w1[n][2 * l + 1] = w * fi;
             }
         }
     }
@@ -686,7 +719,7 @@ __device__ __forceinline__ void tile_to_slot(const VT *tile, cplx_t<typename L::
     using T = typename L::Real;
     using C = cplx_t<T>;
     constexpr int CPL = L::CPL;
-    constexpr uint32_t EOFF = COMP ? (L::CELLS / CPL + (L::NEED_DUMMY ? 64 : 0)) * 16 : 0;
+    constexpr uint32_t EOFF = COMP ? (L::CELLS / CPL + L::DUMMY_UNITS) * 16 : 0;
     const int tid = threadIdx.x;
     const C *tc = reinterpret_cast<const C *>(tile);
 #pragma unroll
@@ -736,7 +769,7 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
     constexpr int NT = NW * 64;
     constexpr int PB = L::PB;
     constexpr bool DUMMY = L::NEED_DUMMY;
-    constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
+    constexpr int TILE_UNITS = L::CELLS / CPL + L::DUMMY_UNITS;
     constexpr int NB = batch_nodes<T, D, W>(TILE_UNITS * 16 * (COMP ? 2 : 1), NW, 0);
     // COMP: a second tile of Kahan compensation terms follows the sums
     constexpr uint32_t EOFF = COMP ? TILE_UNITS * 16 : 0;
@@ -762,7 +795,7 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
 #pragma unroll
     for (int i = 0; i < L::NI; ++i) it[i] = lane_item<L>(i, lane);
     char *tb = reinterpret_cast<char *>(tile);
-    const uint32_t dummy = (uint32_t)(L::CELLS / CPL + lane) * 16;
+    const uint32_t dummy = L::dummy_byte(lane);
     WinBatch<T, D, W, NB> &wb = wbs[wave];
 
     for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
@@ -894,7 +927,7 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
     constexpr int NI = L::NI;
     constexpr int OWN = L::OWN;
     constexpr bool DUMMY = L::NEED_DUMMY;
-    constexpr int TILE_UNITS = L::CELLS / CPL + (DUMMY ? 64 : 0);
+    constexpr int TILE_UNITS = L::CELLS / CPL + L::DUMMY_UNITS;
     constexpr int NB = batch_nodes<T, D, W, NW, 32>(TILE_UNITS * 16, 1, 0);
     using WB = WinBatch<T, D, W, NB, NW>;
     __shared__ __align__(16) VT tile[TILE_UNITS];
@@ -928,7 +961,7 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
         offb[i] = (uint32_t)it.off * L::CB;
     }
     char *tb = reinterpret_cast<char *>(tile);
-    const uint32_t dummy = (uint32_t)(L::CELLS / CPL + lane) * 16;
+    const uint32_t dummy = L::dummy_byte(lane);
 
     VT v[NI][OWN];          // the held units: plane p_cur + NW i of lane item ii
     uint32_t ad[NI][OWN];
@@ -937,8 +970,7 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
     for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
         const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
         if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch
-        const NodeRec rc = batch_prologue_wg<T, D, W, NB, L, NW, NW>(wb, nb, lane, wave, nd, wc);
-        const C fc = fv;
+        const NodeRec rc = batch_prologue_wg<T, D, W, NB, L, NW, NW>(wb, nb, lane, wave, nd, fv, wc);
         {
             const uint32_t s = s0 + NB + lane;
             if (lane < NB && s < ch.count) {
@@ -963,7 +995,6 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
         const T *w2base = &wb.w2[0][0];
         for (int k = 0; k < nb; ++k) {
             const int key = __builtin_amdgcn_readfirstlane(rlane_i(nkey, k));
-            const T fr = rlane_t(fc.x, k), fi = rlane_t(fc.y, k);
             if (key != cur_key) {
                 // write back the held units, then read the node's (in order per wavefront)
                 if (cur_key >= 0) {
@@ -1011,14 +1042,15 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
             const T *w2k = w2base + rlane_i(nw2, k);
 #pragma unroll
             for (int ii = 0; ii < NI; ++ii) {
-                const T w1v = wb.w1[k][ur[ii]];
-                const T wu = w1v * w2k[uc[ii]];
-                const T fux = fr * wu, fuy = fi * wu;
+                // row weight times the coefficient, then the column weights
+                const T w1r = wb.w1[k][2 * ur[ii]], w1i = wb.w1[k][2 * ur[ii] + 1];
+                const T w2a = w2k[uc[ii]];
+                const T fux = w1r * w2a, fuy = w1i * w2a;
                 T fvx = 0, fvy = 0;
                 if constexpr (CPL == 2) {
-                    const T wv = w1v * w2k[uc[ii] + 1];
-                    fvx = fr * wv;
-                    fvy = fi * wv;
+                    const T w2b = w2k[uc[ii] + 1];
+                    fvx = w1r * w2b;
+                    fvy = w1i * w2b;
                 }
 #pragma unroll
                 for (int i = 0; i < OWN; ++i) {
@@ -1450,7 +1482,7 @@ template <typename T, int D, int W, int V> constexpr bool fits_comp()
 {
     using LS = TileLayout<T, D, W, Waves<T, D, W, V>::spread, V>;
     return sizeof(T) == 4 && fits_all<T, D, W, V>() &&
-           2L * (LS::CELLS * LS::CB + (LS::NEED_DUMMY ? 1024 : 0)) <= kLdsMax;
+           2L * (LS::CELLS * LS::CB + LS::DUMMY_UNITS * 16) <= kLdsMax;
 }
 
 // Runs f.template run<D, W, V>() for the runtime window width, dimension and

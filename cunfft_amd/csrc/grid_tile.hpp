@@ -702,14 +702,6 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather_batch(const GridJob<T> 
 }
 
 // ------------------------------------------------------------ spread ----
-// NW wavefronts share the tile; wavefront w owns the planes p0 = w (mod NW),
-// so wavefronts never touch the same cell, and every wavefront walks all the
-// chunk's nodes in order (each wavefront keeps its own weight batch).  The
-// cells of one node are disjoint, so a node's reads are all issued before its
-// adds and writes; consecutive nodes of one wavefront may overlap, so their
-// read-add-write sequences stay in program order.  When a plane group can
-// reach past the window, the lanes concerned read and write a private dummy
-// unit past the tile.  fs holds the coefficients in sorted node order.
 // Padded tile (LDS) -> the chunk's slot, destination-blocked: the block that
 // lands on a neighbouring tile is contiguous.  COMP: the slot receives the
 // sum minus its Kahan term (the second tile at EOFF).
@@ -757,6 +749,14 @@ __device__ __forceinline__ void tile_to_slot(const VT *tile, cplx_t<typename L::
             }
 }
 
+// NW wavefronts share the tile; wavefront w owns the planes p0 = w (mod NW),
+// so wavefronts never touch the same cell, and every wavefront walks all the
+// chunk's nodes in order (each wavefront keeps its own weight batch).  The
+// cells of one node are disjoint, so a node's reads are all issued before its
+// adds and writes; consecutive nodes of one wavefront may overlap, so their
+// read-add-write sequences stay in program order.  When a plane group can
+// reach past the window, the lanes concerned read and write a private dummy
+// unit (TileLayout::dummy_byte).  fs holds the coefficients in sorted node order.
 template <typename T, int D, int W, int NW, int V, bool COMP>
 __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc,
                                             const cplx_t<T> *__restrict__ fs, const Chunk ch,

@@ -60,113 +60,195 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, int64_t n, uint32
     if (i < n) p[i] = v;
 }
 
+// Tile key of every node, its count, and (plane-tiled kernels) its window
+// descriptor by node index: the FP64 index/fraction split of window_start,
+// lo_t the window start relative to the origin of the node's tile, in
+// [0, T_t) by construction of the key.
+template <typename T>
 __global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
                                                   uint32_t *__restrict__ keys,
-                                                  uint32_t *__restrict__ sub,
+                                                  NodeDesc<T> *__restrict__ desc_j,
                                                   uint32_t *__restrict__ tile_count, Geom ge,
                                                   BinGeom bg)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= ge.M) return;
-    uint32_t key = 0;
-    uint32_t lo[3] = {0, 0, 0};
+    uint32_t key = 0, loc = 0;
+    NodeDesc<T> nd{};
     for (int t = 0; t < bg.d; ++t) {
         int64_t u;
         double c;
         window_start(x[j * bg.d + t], ge.n[t], ge.m, u, c);
         const int64_t w = wrap_lo(u, ge.n[t]);
         const uint32_t tc = (uint32_t)(w >> bg.logT[t]);
-        lo[t] = (uint32_t)(w - ((int64_t)tc << bg.logT[t]));
+        loc |= (uint32_t)(w - ((int64_t)tc << bg.logT[t])) << (8 * t);
+        nd.f[t] = (T)(c - (double)ge.m);
         key = key * (uint32_t)bg.nt[t] + tc;
     }
     keys[j] = key;
-    if (sub) sub[j] = sub_key(bg.d, lo, bg.logT, bg.sub_shift);
+    if (desc_j) {
+        nd.loc = loc;
+        desc_j[j] = nd;
+    }
     atomicAdd(&tile_count[key], 1u);
 }
 
-// One workgroup: exclusive scans of the tile counts and of the chunk counts.
-__global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ cnt,
-                                                   uint32_t *__restrict__ start,
-                                                   uint32_t *__restrict__ cursor,
-                                                   uint32_t *__restrict__ chunk_start,
-                                                   uint32_t *__restrict__ n_chunks, int64_t ntiles)
+// Exclusive scans of the tile counts and of the chunk counts, in three
+// launches: sums per block of kScanBlock tiles, one workgroup scanning those
+// sums, then each block's own scan plus its offset.
+constexpr int kScanBlock = 1024;
+
+// Inclusive scan of (a, b) over a workgroup of kScanBlock threads: wavefront
+// scans with shuffles, then the wavefront totals through LDS.
+__device__ __forceinline__ void block_scan2(uint32_t &a, uint32_t &b, uint32_t *wa, uint32_t *wb)
 {
-    __shared__ uint32_t sa[1024], sb[1024];
-    const int tid = threadIdx.x;
-    const int64_t per = (ntiles + 1023) / 1024;
-    const int64_t lo = tid * per, hi = lo + per < ntiles ? lo + per : ntiles;
-    uint32_t a = 0, b = 0;
-    for (int64_t i = lo; i < hi; ++i) {
-        a += cnt[i];
-        b += (cnt[i] + kChunk - 1) / kChunk;
-    }
-    sa[tid] = a;
-    sb[tid] = b;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t va = 0, vb = 0;
-        if (tid >= off) {
-            va = sa[tid - off];
-            vb = sb[tid - off];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t va = __shfl_up(a, off, 64), vb = __shfl_up(b, off, 64);
+        if (lane >= off) {
+            a += va;
+            b += vb;
         }
-        __syncthreads();
-        sa[tid] += va;
-        sb[tid] += vb;
-        __syncthreads();
     }
-    uint32_t ra = sa[tid] - a, rb = sb[tid] - b;
-    for (int64_t i = lo; i < hi; ++i) {
-        const uint32_t c = cnt[i];
-        start[i] = ra;
-        chunk_start[i] = rb;
-        cursor[i] = 0;
-        ra += c;
-        rb += (c + kChunk - 1) / kChunk;
+    if (lane == 63) {
+        wa[wave] = a;
+        wb[wave] = b;
     }
-    if (tid == 1023) n_chunks[0] = sb[1023];
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t ta = lane < kScanBlock / 64 ? wa[lane] : 0u, tb = lane < kScanBlock / 64 ? wb[lane] : 0u;
+        for (int off = 1; off < kScanBlock / 64; off <<= 1) {
+            const uint32_t va = __shfl_up(ta, off, 64), vb = __shfl_up(tb, off, 64);
+            if (lane >= off) {
+                ta += va;
+                tb += vb;
+            }
+        }
+        if (lane < kScanBlock / 64) {
+            wa[lane] = ta;
+            wb[lane] = tb;
+        }
+    }
+    __syncthreads();
+    if (wave > 0) {
+        a += wa[wave - 1];
+        b += wb[wave - 1];
+    }
 }
 
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan_part(const uint32_t *__restrict__ cnt,
+                                                              uint32_t *__restrict__ part,
+                                                              int64_t ntiles, int64_t nparts)
+{
+    __shared__ uint32_t wa[kScanBlock / 64], wb[kScanBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const uint32_t c = i < ntiles ? cnt[i] : 0u;
+    uint32_t a = c, b = (c + kChunk - 1) / kChunk;
+    block_scan2(a, b, wa, wb);
+    if (threadIdx.x == kScanBlock - 1) {
+        part[blockIdx.x] = a;
+        part[nparts + blockIdx.x] = b;
+    }
+}
+
+// One workgroup: exclusive scan of the block sums in place; the chunk total.
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan_top(uint32_t *__restrict__ part,
+                                                             int64_t nparts,
+                                                             uint32_t *__restrict__ n_chunks)
+{
+    __shared__ uint32_t wa[kScanBlock / 64], wb[kScanBlock / 64];
+    uint32_t ca = 0, cb = 0;
+    for (int64_t base = 0; base < nparts; base += kScanBlock) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t a0 = i < nparts ? part[i] : 0u, b0 = i < nparts ? part[nparts + i] : 0u;
+        uint32_t a = a0, b = b0;
+        block_scan2(a, b, wa, wb);
+        if (i < nparts) {
+            part[i] = ca + a - a0;
+            part[nparts + i] = cb + b - b0;
+        }
+        __syncthreads();   // every thread has its block totals before wa/wb are reused
+        ca += wa[kScanBlock / 64 - 1];
+        cb += wb[kScanBlock / 64 - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) n_chunks[0] = cb;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan_apply(const uint32_t *__restrict__ cnt,
+                                                               const uint32_t *__restrict__ part,
+                                                               uint32_t *__restrict__ start,
+                                                               uint32_t *__restrict__ cursor,
+                                                               uint32_t *__restrict__ chunk_start,
+                                                               int64_t ntiles, int64_t nparts)
+{
+    __shared__ uint32_t wa[kScanBlock / 64], wb[kScanBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const uint32_t c = i < ntiles ? cnt[i] : 0u, cc = (c + kChunk - 1) / kChunk;
+    uint32_t a = c, b = cc;
+    block_scan2(a, b, wa, wb);
+    if (i < ntiles) {
+        start[i] = part[blockIdx.x] + a - c;
+        chunk_start[i] = part[nparts + blockIdx.x] + b - cc;
+        cursor[i] = 0;
+    }
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_bin_place(const double *__restrict__ x,
                                                    const uint32_t *__restrict__ keys,
                                                    const uint32_t *__restrict__ start,
                                                    uint32_t *__restrict__ cursor,
                                                    uint32_t *__restrict__ perm,
                                                    double *__restrict__ xs,
-                                                   const uint32_t *__restrict__ sub,
-                                                   uint32_t *__restrict__ sub_out, int64_t M, int d)
+                                                   const NodeDesc<T> *__restrict__ desc_j,
+                                                   NodeDesc<T> *__restrict__ desc_out, int64_t M, int d)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= M) return;
     const uint32_t k = keys[j];
     const uint32_t pos = start[k] + atomicAdd(&cursor[k], 1u);
     perm[pos] = (uint32_t)j;
-    if (sub) sub_out[pos] = sub[j];
+    if (desc_j) desc_out[pos] = desc_j[j];
     if (xs)
         for (int t = 0; t < d; ++t) xs[(int64_t)pos * d + t] = x[j * d + t];
 }
 
 // Second pass of the sort, one wavefront per tile: a counting sort of the
-// tile's nodes on sub_key (LDS histogram of nrange bins, wavefront scan,
-// placement), from the placement order perm_in (keys sub_in, in the same
-// order) to perm_out.  Equal keys keep no particular order, as in the
-// placement.  nrange is a power of two >= 64.
+// tile's nodes on sub_key of their window start (LDS histogram of nrange
+// bins, wavefront scan, placement), from the placement order (perm_in,
+// desc_in) to the final one (perm_out, desc_out).  Equal keys keep no
+// particular order, as in the placement.  nrange is a power of two >= 64.
+template <typename T>
+__device__ __forceinline__ uint32_t desc_sub_key(const NodeDesc<T> &nd, const BinGeom &bg)
+{
+    const uint32_t loc = (uint32_t)nd.loc;
+    const uint32_t lo[3] = {loc & 255, (loc >> 8) & 255, (loc >> 16) & 255};
+    return sub_key(bg.d, lo, bg.logT, bg.sub_shift);
+}
+
+template <typename T>
 __global__ __launch_bounds__(64) void k_bin_subsort(const uint32_t *__restrict__ cnt,
                                                     const uint32_t *__restrict__ start,
-                                                    const uint32_t *__restrict__ sub_in,
                                                     const uint32_t *__restrict__ perm_in,
+                                                    const NodeDesc<T> *__restrict__ desc_in,
                                                     uint32_t *__restrict__ perm_out,
-                                                    uint32_t nrange)
+                                                    NodeDesc<T> *__restrict__ desc_out,
+                                                    uint32_t nrange, BinGeom bg)
 {
     extern __shared__ uint32_t hist[];
     const uint32_t tile = blockIdx.x, lane = threadIdx.x;
     const uint32_t c = cnt[tile], s0 = start[tile];
     if (c <= 1) {
-        if (c == 1 && lane == 0) perm_out[s0] = perm_in[s0];
+        if (c == 1 && lane == 0) {
+            perm_out[s0] = perm_in[s0];
+            desc_out[s0] = desc_in[s0];
+        }
         return;
     }
     for (uint32_t i = lane; i < nrange; i += 64) hist[i] = 0;
     __syncthreads();
-    for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[sub_in[s0 + i]], 1u);
+    for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[desc_sub_key(desc_in[s0 + i], bg)], 1u);
     __syncthreads();
     // exclusive scan: lane owns bins [lane * per, (lane + 1) * per)
     const uint32_t per = nrange / 64, b0 = lane * per;
@@ -184,41 +266,12 @@ __global__ __launch_bounds__(64) void k_bin_subsort(const uint32_t *__restrict__
         run += h;
     }
     __syncthreads();
-    for (uint32_t i = lane; i < c; i += 64)
-        perm_out[s0 + atomicAdd(&hist[sub_in[s0 + i]], 1u)] = perm_in[s0 + i];
-}
-
-// Window descriptor of every sorted node (NodeDesc).  The index/fraction split
-// is the FP64 one of window_start; lo_t is the window start relative to the
-// origin of the node's tile, in [0, T_t) by construction of the key.
-template <typename T>
-__global__ __launch_bounds__(256) void k_bin_desc(const double *__restrict__ x,
-                                                  const uint32_t *__restrict__ perm,
-                                                  const uint32_t *__restrict__ keys,
-                                                  NodeDesc<T> *__restrict__ desc, Geom ge,
-                                                  BinGeom bg)
-{
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= ge.M) return;
-    const uint32_t j = perm[s];
-    uint32_t rem = keys[j];
-    uint32_t tc[3] = {0, 0, 0};
-    for (int t = bg.d - 1; t >= 0; --t) {
-        tc[t] = rem % (uint32_t)bg.nt[t];
-        rem /= (uint32_t)bg.nt[t];
+    for (uint32_t i = lane; i < c; i += 64) {
+        const NodeDesc<T> nd = desc_in[s0 + i];
+        const uint32_t pos = s0 + atomicAdd(&hist[desc_sub_key(nd, bg)], 1u);
+        perm_out[pos] = perm_in[s0 + i];
+        desc_out[pos] = nd;
     }
-    NodeDesc<T> nd{};
-    uint32_t loc = 0;
-    for (int t = 0; t < bg.d; ++t) {
-        int64_t u;
-        double c;
-        window_start(x[(int64_t)j * bg.d + t], ge.n[t], ge.m, u, c);
-        const uint32_t lo = (uint32_t)(wrap_lo(u, ge.n[t]) - ((int64_t)tc[t] << bg.logT[t]));
-        loc |= lo << (8 * t);
-        nd.f[t] = (T)(c - (double)ge.m);
-    }
-    nd.loc = loc;
-    desc[s] = nd;
 }
 
 __global__ __launch_bounds__(256) void k_bin_chunks(const uint32_t *__restrict__ cnt,
@@ -286,6 +339,34 @@ static void *dev_alloc(size_t bytes)
     return p;
 }
 
+// The binning passes: keys (and, for the plane-tiled kernels, descriptors by
+// node), tile scan, placement, chunk list, and the sort within tiles.
+template <typename T> void Binning::run_build(const Geom &ge, const BinGeom &bg, const double *x,
+                                              hipStream_t s)
+{
+    const unsigned gm = (unsigned)((M + 255) / 256);
+    const int64_t nparts = (ntiles + kScanBlock - 1) / kScanBlock;
+    NodeDesc<T> *dj = planes() ? static_cast<NodeDesc<T> *>(desc) : nullptr;   // by node until placed
+    NodeDesc<T> *dp = planes() ? static_cast<NodeDesc<T> *>(desc2) : nullptr;  // placement order
+    k_fill_u32<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, ntiles, 0u);
+    k_bin_keys<T><<<gm, 256, 0, s>>>(x, keys, dj, tile_count, ge, bg);
+    k_bin_scan_part<<<(unsigned)nparts, kScanBlock, 0, s>>>(tile_count, scan_part, ntiles, nparts);
+    k_bin_scan_top<<<1, kScanBlock, 0, s>>>(scan_part, nparts, n_chunks);
+    k_bin_scan_apply<<<(unsigned)nparts, kScanBlock, 0, s>>>(tile_count, scan_part, tile_start, cursor,
+                                                            chunk_start, ntiles, nparts);
+    k_bin_place<T><<<gm, 256, 0, s>>>(x, keys, tile_start, cursor, planes() ? perm2 : perm, xs, dj,
+                                      dp, M, d);
+    k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
+                                                                  chunk_start, chunks, ntiles);
+    if (planes()) {
+        const uint32_t nr = sub_range(bg);
+        if (nr < 64 || (nr & (nr - 1)) != 0 || nr > 4096)
+            throw Error(CUNFFT_ERR_UNSUPPORTED, "tile sub-key range must be a power of two in [64, 4096]");
+        k_bin_subsort<T><<<(unsigned)ntiles, 64, nr * sizeof(uint32_t), s>>>(
+            tile_count, tile_start, perm2, dp, perm, static_cast<NodeDesc<T> *>(desc), nr, bg);
+    }
+}
+
 void Binning::build(const Geom &ge, const double *x, hipStream_t s)
 {
     if (!usable) return;
@@ -295,11 +376,8 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
         max_chunks = ntiles + (M + kChunk - 1) / kChunk;
         keys = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
         perm = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-        if (planes()) {
-            sub = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-            sub2 = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-            perm2 = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-        }
+        if (planes()) perm2 = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
+        scan_part = (uint32_t *)dev_alloc(sizeof(uint32_t) * 2 * ((ntiles + kScanBlock - 1) / kScanBlock));
         tile_count = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
         tile_start = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
         cursor = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
@@ -307,8 +385,9 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
         n_chunks = (uint32_t *)dev_alloc(sizeof(uint32_t));
         chunks = (Chunk *)dev_alloc(sizeof(Chunk) * max_chunks);
         if (planes()) {
-            desc = dev_alloc((size_t)M * (elem_bytes == 4 ? sizeof(NodeDesc<float>)
-                                                          : sizeof(NodeDesc<double>)));
+            const size_t db = elem_bytes == 4 ? sizeof(NodeDesc<float>) : sizeof(NodeDesc<double>);
+            desc = dev_alloc((size_t)M * db);
+            desc2 = dev_alloc((size_t)M * db);
             if (own_scratch) {
                 slots = dev_alloc((size_t)max_chunks * slot_cells * 2 * elem_bytes);
                 fs = dev_alloc((size_t)M * 2 * elem_bytes);
@@ -318,25 +397,8 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
         }
     }
     const BinGeom bg = bin_geom(*this);
-    const unsigned gm = (unsigned)((M + 255) / 256);
-    k_fill_u32<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, ntiles, 0u);
-    k_bin_keys<<<gm, 256, 0, s>>>(x, keys, planes() ? sub : nullptr, tile_count, ge, bg);
-    k_bin_scan<<<1, 1024, 0, s>>>(tile_count, tile_start, cursor, chunk_start, n_chunks, ntiles);
-    k_bin_place<<<gm, 256, 0, s>>>(x, keys, tile_start, cursor, planes() ? perm2 : perm, xs,
-                                   planes() ? sub : nullptr, sub2, M, d);
-    k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
-                                                                  chunk_start, chunks, ntiles);
-    if (planes()) {
-        const uint32_t nr = sub_range(bg);
-        if (nr < 64 || (nr & (nr - 1)) != 0 || nr > 4096)
-            throw Error(CUNFFT_ERR_UNSUPPORTED, "tile sub-key range must be a power of two in [64, 4096]");
-        k_bin_subsort<<<(unsigned)ntiles, 64, nr * sizeof(uint32_t), s>>>(
-            tile_count, tile_start, sub2, perm2, perm, nr);
-        if (elem_bytes == 4)
-            k_bin_desc<float><<<gm, 256, 0, s>>>(x, perm, keys, (NodeDesc<float> *)desc, ge, bg);
-        else
-            k_bin_desc<double><<<gm, 256, 0, s>>>(x, perm, keys, (NodeDesc<double> *)desc, ge, bg);
-    }
+    if (elem_bytes == 4) run_build<float>(ge, bg, x, s);
+    else run_build<double>(ge, bg, x, s);
     CUNFFT_HIP(hipGetLastError());
     built = 1;
 }
@@ -344,8 +406,8 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
 void Binning::release()
 {
     if (keys) (void)hipFree(keys);
-    if (sub) (void)hipFree(sub);
-    if (sub2) (void)hipFree(sub2);
+    if (scan_part) (void)hipFree(scan_part);
+    if (desc2) (void)hipFree(desc2);
     if (perm) (void)hipFree(perm);
     if (perm2) (void)hipFree(perm2);
     if (tile_count) (void)hipFree(tile_count);
@@ -358,7 +420,8 @@ void Binning::release()
     if (desc) (void)hipFree(desc);
     if (slots) (void)hipFree(slots);
     if (fs) (void)hipFree(fs);
-    keys = sub = sub2 = perm = perm2 = tile_count = tile_start = cursor = chunk_start = n_chunks = nullptr;
+    keys = scan_part = perm = perm2 = tile_count = tile_start = cursor = chunk_start = n_chunks = nullptr;
+    desc2 = nullptr;
     chunks = nullptr;
     xs = nullptr;
     desc = nullptr;

@@ -56,6 +56,8 @@ template <> struct alignas(16) NodeDesc<double> {
     uint64_t loc;
 };
 
+struct BinGeom;
+
 // Nodes sorted (counting sort on the device) by the grid tile that holds the
 // start of their window, u_t mod n_t.  Built once per node set on the plan
 // stream and reused by every forward and adjoint call until the nodes change.
@@ -78,8 +80,7 @@ struct Binning {
     int64_t max_chunks = 0;
     int64_t slot_cells = 0;     // padded-tile cells of one adjoint slot
     uint32_t *keys = nullptr;       // M: tile of each node
-    uint32_t *sub = nullptr;        // M: window start within the tile (plane-tiled kernels)
-    uint32_t *sub2 = nullptr;       // M: the same in placement order
+    uint32_t *scan_part = nullptr;  // 2 x ceil(ntiles / 1024): block sums of the tile scan
     uint32_t *perm = nullptr;       // M: node at each sorted position
     uint32_t *perm2 = nullptr;      // M: placement order before the sort within tiles
     uint32_t *tile_count = nullptr; // ntiles
@@ -90,6 +91,7 @@ struct Binning {
     Chunk *chunks = nullptr;        // max_chunks
     double *xs = nullptr;           // M*d: nodes in sorted order (node-lane kernels)
     void *desc = nullptr;           // M: NodeDesc<T> in sorted order (plane-tiled kernels)
+    void *desc2 = nullptr;          // M: NodeDesc<T> in placement order (binning scratch)
     void *slots = nullptr;          // max_chunks padded tiles (adjoint partial sums)
     void *fs = nullptr;             // M: adjoint coefficients in sorted order
     int elem_bytes = 8;             // bytes of one real component (4: FP32 plan, 8: FP64)
@@ -99,6 +101,7 @@ struct Binning {
     // Decide tile shape for this geometry; false if the tiled path does not apply.
     bool plan(const Geom &ge, int elem_bytes, int kernel_variant, int tile_variant = 0);
     void build(const Geom &ge, const double *x, hipStream_t s);
+    template <typename T> void run_build(const Geom &ge, const BinGeom &bg, const double *x, hipStream_t s);
     void release();
     bool planes() const { return d >= 2 && variant == 0; }
 };

@@ -724,6 +724,37 @@ __device__ __forceinline__ void tile_to_slot(const VT *tile, cplx_t<typename L::
                           EL = L::ext(L::TL, L::PL, qL);
                 const int n = E0 * E1 * EL;
                 C *dst = slot + L::block_off(q0, q1, qL);
+                if (CPL == 2 && EL % 2 == 0) {
+                    // two cells per 16-byte move: block rows and padded rows start
+                    // on even cells (even pitch, even tile extents, even slot size)
+                    const int np = n / 2;
+                    for (int c0 = 0; c0 < np; c0 += 8 * NT) {
+                        VT v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            int c = c0 + tid + u * NT;
+                            c = 2 * (c < np ? c : np - 1);
+                            const int r0 = c / (E1 * EL), rr = c - r0 * (E1 * EL);
+                            const int r1 = rr / EL, rL = rr - r1 * EL;
+                            const int src = (q0 * L::T0 + r0) * L::PP + (q1 * L::T1 + r1) * L::RP +
+                                            qL * L::TL + rL;
+                            v[u] = tile[src / 2];
+                            if constexpr (COMP) {
+                                const VT e = tile[src / 2 + EOFF / 16];
+                                v[u].x -= e.x;
+                                v[u].y -= e.y;
+                                v[u].z -= e.z;
+                                v[u].w -= e.w;
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int c = c0 + tid + u * NT;
+                            if (c < np) reinterpret_cast<VT *>(dst)[c] = v[u];
+                        }
+                    }
+                    continue;
+                }
                 for (int c0 = 0; c0 < n; c0 += 8 * NT) {
                     C v[8];
 #pragma unroll

@@ -1311,10 +1311,10 @@ __device__ __forceinline__ void store_pair(double2 *p, double2 a, double2 b)
 // by (q0, q1): the first chunks of the q_L blocks in flight together, then the
 // further chunks of dense tiles.  The order of the sums is fixed, so the grid
 // is reproducible for a given binning.
-// The column-segment fold serves FP32 3D grids (0.51 -> 0.44 ms at 256^3); for
+// The column-segment fold serves FP32 grids (3D 256^3: 0.51 -> 0.44 ms); for
 // FP64 the per-cell fold (tile_fold), whose loads are already 16 bytes per lane and
 // contiguous over the wavefront, measured faster (0.74 vs 0.78 ms).
-template <typename T, int D> constexpr bool kFoldColumns = D == 3 && sizeof(T) == 4;
+template <typename T, int D> constexpr bool kFoldColumns = sizeof(T) == 4;
 
 template <typename L> struct Fold3Shape {
     static constexpr int SEGC = 64 / L::CB;                 // cells per thread
@@ -1322,39 +1322,45 @@ template <typename L> struct Fold3Shape {
     static constexpr int TPT = L::T0 * L::T1 * S;           // threads per tile
     static constexpr int TPW = TPT <= 256 ? 256 / TPT : 1;  // tiles per workgroup
     static constexpr int WPT = TPT > 256 ? TPT / 256 : 1;   // workgroups per tile
-    static_assert(L::TL % SEGC == 0 && TPT % 64 == 0 && (256 % TPT == 0 || TPT % 256 == 0),
-                  "column segments tile the core and whole wavefronts");
+    static_assert(L::TL % SEGC == 0 && (256 % TPT == 0 || TPT % 256 == 0),
+                  "column segments tile the core and whole workgroups");
 };
 
-template <typename T, int W, int PS, int V, bool COMP>
-__device__ __forceinline__ void tile_fold3(const cplx_t<T> *__restrict__ slots,
-                                           const uint32_t *__restrict__ tile_count,
-                                           const uint32_t *__restrict__ chunk_start,
-                                           cplx_t<T> *__restrict__ g, const Geom &ge,
-                                           const BinGeom &bg, uint32_t blk)
+template <typename T, int D, int W, int PS, int V, bool COMP>
+__device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slots,
+                                               const uint32_t *__restrict__ tile_count,
+                                               const uint32_t *__restrict__ chunk_start,
+                                               cplx_t<T> *__restrict__ g, const Geom &ge,
+                                               const BinGeom &bg, uint32_t blk)
 {
-    using L = TileLayout<T, 3, W, PS, V>;
+    using L = TileLayout<T, D, W, PS, V>;
     using F = Fold3Shape<L>;
     using C = cplx_t<T>;
     constexpr int TL = L::TL, SEGC = F::SEGC, S = F::S;
     constexpr int ELAST = L::PL - (L::QL - 1) * TL;          // extent of the last q_L block
     constexpr int NLAST = ELAST < SEGC ? ELAST : SEGC;       // cells a thread loads there
-    static_assert(L::T0 % 2 == 0 && L::T1 % 2 == 0 && TL % 2 == 0,
+    static_assert(L::T0 % 2 == 0 && (D == 2 || L::T1 % 2 == 0) && TL % 2 == 0,
                   "even tile extents keep slot rows and grid rows 16-byte aligned");
     const int tid = threadIdx.x;
     const uint32_t tile = F::TPT <= 256 ? blk * F::TPW + (uint32_t)(tid / F::TPT) : blk / F::WPT;
-    if (tile >= (uint32_t)bg.ntiles) return;   // whole wavefronts
+    if (tile >= (uint32_t)bg.ntiles) return;   // no barriers follow
     const int tt = F::TPT <= 256 ? tid % F::TPT : (int)(blk % F::WPT) * 256 + tid;
     const int col = tt / S, seg = tt % S;
     const int r0 = col / L::T1, r1 = col % L::T1;
-    uint32_t tk[3];
+    // tile coordinates on axes 0, 1 (d = 3; 0 and 0 for d = 2) and L
+    uint32_t tk[3] = {0, 0, 0};
     {
         uint32_t rem = tile;
-        for (int t = 2; t >= 0; --t) {
-            tk[t] = rem % (uint32_t)bg.nt[t];
-            rem /= (uint32_t)bg.nt[t];
+        tk[2] = rem % (uint32_t)bg.nt[D - 1];
+        rem /= (uint32_t)bg.nt[D - 1];
+        if (D == 3) {
+            tk[1] = rem % (uint32_t)bg.nt[1];
+            rem /= (uint32_t)bg.nt[1];
         }
+        tk[0] = rem;
     }
+    const uint32_t n0 = (uint32_t)bg.nt[0], n1 = D == 3 ? (uint32_t)bg.nt[1] : 1u,
+                   n2 = (uint32_t)bg.nt[D - 1];
     T sx[SEGC], sy[SEGC], cx[SEGC], cy[SEGC];
 #pragma unroll
     for (int i = 0; i < SEGC; ++i) sx[i] = sy[i] = cx[i] = cy[i] = 0;
@@ -1384,7 +1390,6 @@ __device__ __forceinline__ void tile_fold3(const cplx_t<T> *__restrict__ slots,
             C v[QL][SEGC];
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
-                const uint32_t n0 = (uint32_t)bg.nt[0], n1 = (uint32_t)bg.nt[1], n2 = (uint32_t)bg.nt[2];
                 const uint32_t src = ((tk[0] + 4 * n0 - q0) % n0 * n1 + (tk[1] + 4 * n1 - q1) % n1) * n2 +
                                      (tk[2] + 4 * n2 - qL) % n2;
                 const uint32_t cnt = tile_count[src];
@@ -1430,8 +1435,8 @@ __device__ __forceinline__ void tile_fold3(const cplx_t<T> *__restrict__ slots,
                 }
             }
         }
-    const uint32_t n1 = (uint32_t)ge.n[1], n2 = (uint32_t)ge.n[2];
-    C *dst = g + (size_t)((tk[0] * L::T0 + r0) * n1 + tk[1] * L::T1 + r1) * n2 + tk[2] * TL + seg * SEGC;
+    const uint32_t g1 = D == 3 ? (uint32_t)ge.n[1] : 1u, gL = (uint32_t)ge.n[D - 1];
+    C *dst = g + (size_t)((tk[0] * L::T0 + r0) * g1 + tk[1] * L::T1 + r1) * gL + tk[2] * TL + seg * SEGC;
 #pragma unroll
     for (int i = 0; i < SEGC / 2; ++i)
         store_pair(dst + 2 * i, cmake<T>(sx[2 * i] + cx[2 * i], sy[2 * i] + cy[2 * i]),
@@ -1445,7 +1450,7 @@ __global__ __launch_bounds__(256) void k_tile_fold(const cplx_t<T> *__restrict__
                                                    cplx_t<T> *__restrict__ g, Geom ge, BinGeom bg)
 {
     if constexpr (kFoldColumns<T, D>)
-        tile_fold3<T, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x);
+        tile_fold_cols<T, D, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x);
     else
         tile_fold<T, D, W, PS, V, COMP>(slots, tile_count, chunk_start, g, ge, bg, blockIdx.x,
                                         blockIdx.y);
@@ -1462,7 +1467,7 @@ __global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__res
     const BlockMap bm = map[blockIdx.x];
     const GridJob<T> &J = jobs[bm.job];
     if constexpr (kFoldColumns<T, D>)
-        tile_fold3<T, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg, bm.blk);
+        tile_fold_cols<T, D, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg, bm.blk);
     else
         tile_fold<T, D, W, PS, V, false>(J.slots, J.tile_count, J.chunk_start, J.g, J.ge, J.bg,
                                          bm.blk / SLICES, (int)(bm.blk % SLICES));
@@ -1614,7 +1619,7 @@ template <typename T> struct SpreadOp {
             constexpr int CORE = L::T0 * L::T1 * L::TL;
             constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
             const dim3 fold_grid = kFoldColumns<T, D>
-                                       ? dim3((unsigned)tile_fold_blocks(3, V, b.ntiles, sizeof(T)), 1)
+                                       ? dim3((unsigned)tile_fold_blocks(D, V, b.ntiles, sizeof(T)), 1)
                                        : dim3((unsigned)b.ntiles, SLICES);
             if constexpr (fits_comp<T, D, W, V>()) {
                 if (b.compensated) {

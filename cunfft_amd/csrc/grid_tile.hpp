@@ -1161,9 +1161,10 @@ __global__ __launch_bounds__(256) void k_sorted_coeffs_batch(const GridJob<T> *_
 
 // -------------------------------------------------------------- fold ----
 // g on one grid tile = sum over the chunks of the tiles whose padded region
-// covers it of the matching slot block.  One workgroup per tile and slice of
-// 512 core cells (blockIdx.y); each block a
-// thread reads is contiguous over the workgroup.  The first chunk of the
+// covers it of the matching slot block.  Per-cell form (FP64; the FP32 form
+// by column segments follows): one workgroup per tile and slice of 512 core
+// cells (blockIdx.y); each block a thread reads is contiguous over the
+// workgroup.  The first chunk of the
 // candidate tiles of one leading-axis offset is loaded at once (a dense tile's
 // further chunks follow), and the sums run in a fixed order: the result does
 // not depend on scheduling.

@@ -556,8 +556,23 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
         // each would wait out the full memory latency).
         constexpr int ROWS = L::P0 * L::P1, RS = NT / NR, SB = 8;
         static_assert(RS >= 1, "a workgroup covers a padded row");
+        // wrapped grid offset of a padded row; when they fit, all of them are
+        // computed once per workgroup into the weight-batch memory (unused
+        // until the tile is staged)
+        auto row_offset = [&](int row) {
+            const int p0 = row / L::P1, p1 = row - p0 * L::P1;
+            // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
+            uint32_t off = wrap_hi32(o[0] + p0, n0) * s0;
+            if (D == 3) off += wrap_hi32(o[1] + p1, n1) * s1;
+            return off;
+        };
+        constexpr bool TABLE = ROWS * sizeof(uint32_t) <= sizeof(wbs);
+        uint32_t *rowoff = reinterpret_cast<uint32_t *>(&wbs[0]);
+        if constexpr (TABLE) {
+            for (int row = tid; row < ROWS; row += NT) rowoff[row] = row_offset(row);
+            __syncthreads();
+        }
         const int un = tid % NR, r0 = tid / NR;
-        // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
         const uint32_t colc = wrap_hi32(o[D - 1] + CPL * un, nL);
         if (r0 < RS) {
             for (int rb = r0; rb < ROWS; rb += SB * RS) {
@@ -566,11 +581,8 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
 #pragma unroll
                 for (int u = 0; u < SB; ++u) {
                     const int row = rb + u * RS < ROWS ? rb + u * RS : ROWS - 1;
-                    const int p0 = row / L::P1, p1 = row - p0 * L::P1;
-                    uint32_t cell = wrap_hi32(o[0] + p0, n0) * s0 + colc;
-                    if (D == 3) cell += wrap_hi32(o[1] + p1, n1) * s1;
                     at[u] = (row * L::RP) / CPL + un;
-                    v[u] = gv[cell / CPL];
+                    v[u] = gv[((TABLE ? rowoff[row] : row_offset(row)) + colc) / CPL];
                 }
                 // unconditional stores (a thread past the last row rewrites it
                 // with its own value): a conditional store would pull its load

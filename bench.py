@@ -377,6 +377,44 @@ def bench_batch(cu, steps, warmup, seed=5150):
     }
 
 
+def config_sweep(cu, steps):
+    """The FP32 headline workload (same nodes and data as the bench step) in each
+    gridding configuration: `steps` forward and adjoint transforms per
+    configuration after one warm-up, with the per-kernel event times of the last
+    one.  Meant to run under a profiler, whose per-kernel counters then separate
+    the configurations by kernel name (the tile shape is a template argument)."""
+    import torch
+
+    x, fh, fv = make_inputs()
+    p = cu.Plan(N3, M_NODES, n=NN3, m=CUT, precision="single")
+    p.x[:] = x
+    p.f_hat[:] = fh
+    p.f[:] = fv
+    p.copyDataToDevice()
+    p.copyDataToDeviceAd()
+    dp = p.device_pointers
+    out = {}
+    for c in range(cu.TUNE_CONFIGS):
+        try:
+            p.config = c
+        except cu.CunfftError as e:
+            out[c] = f"refused: {e}"
+            continue
+        for _ in range(steps + 1):
+            p.transform_dev(dp["f_hat"], dp["f"])
+            p.adjoint_dev(dp["f"], dp["f_hat"])
+        torch.cuda.synchronize()
+        p._p.flags = p._p.flags | cu.MEASURE_TIME
+        p.transform()
+        tf = dict(p.times)
+        p.adjoint()
+        ta = dict(p.times)
+        p._p.flags = p._p.flags & ~cu.MEASURE_TIME
+        out[c] = {"forward": tf, "adjoint": ta}
+    p.finalize()
+    return {"config_sweep": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,6 +424,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D workload")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched 2D workload (prints its JSON object)")
+    ap.add_argument("--config-sweep", action="store_true",
+                    help="only run the FP32 headline workload in every gridding configuration "
+                         "(for per-configuration profiler counters) and print per-config times")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (to scale HBM "
                          "counters) and exit")
@@ -411,6 +452,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if a.batch_only:
         print(json.dumps(bench_batch(cu, a.steps, a.warmup)), flush=True)
+        return 0
+    if a.config_sweep:
+        print(json.dumps(config_sweep(cu, a.steps)), flush=True)
         return 0
     log("bench: building inputs")
     x, fh, fv = make_inputs()

@@ -516,7 +516,9 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
     using C = cplx_t<T>;
     constexpr int CPL = L::CPL;
     constexpr int NT = NW * 64;
-    constexpr int NR = L::RL / CPL;               // lane units staged per row
+    // lane units staged per row: the whole pitch, spare units included, so the
+    // lanes of a wavefront write consecutive LDS units (no bank conflicts)
+    constexpr int NR = L::RP / CPL;
     constexpr int PB = L::PB;
     constexpr int NB = batch_nodes<T, D, W>(L::CELLS * L::CB, NW, 4 * (int)sizeof(C));
     __shared__ __align__(16) C tile[L::CELLS];

@@ -22,12 +22,11 @@
 namespace cunfft {
 
 constexpr int kFftThreads = 256;
-// Elements C * L per workgroup: up to 8192 in FP32 (so a 512-point column
-// pass still reads 16 columns = 128 contiguous bytes per row), 4096 in FP64.
-// The kernels hold E / 1024 radix-4 butterflies per thread in registers and
-// are instantiated for E = 4096 and (FP32) 8192; a launch takes the smaller
-// one that fits its C * L.
-template <typename T> constexpr int fft_max_elems() { return sizeof(T) == 4 ? 4096 : 4096; }
+// Elements C * L per workgroup: up to 4096 (an 8192-element FP32 variant,
+// for 16 columns of a 512-point column pass, measured slower and was
+// dropped).  The kernels hold E / 1024 radix-4 butterflies per thread in
+// registers.
+template <typename T> constexpr int fft_max_elems() { return 4096; }
 constexpr int kFftSmallElems = 4096;
 // global loads in flight per thread (FP64: fewer, the registers run short)
 template <typename T> constexpr int fft_load_batch() { return sizeof(T) == 4 ? 8 : 4; }

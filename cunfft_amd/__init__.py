@@ -423,7 +423,9 @@ class Plan:
     # ------------------------------------------------------------ autotuner
     def autotune(self, directions=TUNE_FORWARD | TUNE_ADJOINT, reps=5, stream=None):
         """Time the gridding configurations on the plan's current nodes and keep
-        the fastest per direction (cunfft_autotune).  Returns ((forward config,
+        the fastest per direction, or, when both directions are tuned to
+        different ones, the fastest forward + adjoint step of that pair and of
+        either one on both directions (cunfft_autotune).  Returns ((forward config,
         adjoint config), forward times, adjoint times) with times in ms per
         configuration, -1 where a configuration does not apply or a direction
         was not tuned."""

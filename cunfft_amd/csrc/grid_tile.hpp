@@ -716,6 +716,29 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather_batch(const GridJob<T> 
 }
 
 // ------------------------------------------------------------ spread ----
+// Slot stores carry the nontemporal hint: the fold reads each slot once, after
+// the whole spread has written every other slot, so keeping them in the caches
+// only evicts what the fold and the next passes read (measured on gfx950: the
+// 3D FP32 fold 8% faster, the spread unchanged).
+__device__ __forceinline__ void stream_store(float4 *p, float4 v)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<f4 *>(p));
+}
+__device__ __forceinline__ void stream_store(float2 *p, float2 v)
+{
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<f2 *>(p));
+}
+__device__ __forceinline__ void stream_store(double2 *p, double2 v)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<d2 *>(p));
+}
+
 // Padded tile (LDS) -> the chunk's slot, destination-blocked: the block that
 // lands on a neighbouring tile is contiguous.  COMP: the slot receives the
 // sum minus its Kahan term (the second tile at EOFF).
@@ -764,7 +787,7 @@ __device__ __forceinline__ void tile_to_slot(const VT *tile, cplx_t<typename L::
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
                             const int c = c0 + tid + u * NT;
-                            if (c < np) reinterpret_cast<VT *>(dst)[c] = v[u];
+                            if (c < np) stream_store(reinterpret_cast<VT *>(dst) + c, v[u]);
                         }
                     }
                     continue;
@@ -788,7 +811,7 @@ __device__ __forceinline__ void tile_to_slot(const VT *tile, cplx_t<typename L::
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         const int c = c0 + tid + u * NT;
-                        if (c < n) dst[c] = v[u];
+                        if (c < n) stream_store(dst + c, v[u]);
                     }
                 }
             }

@@ -501,7 +501,9 @@ static void run_adjoint(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T
 
 // Times every configuration that applies on the plan's current nodes (best of
 // reps forward and adjoint passes over constant data, event-timed on s) and keeps
-// the fastest per tuned direction.  The binning of each candidate is built
+// the fastest per tuned direction, then, with both directions tuned to
+// different configurations, the fastest forward + adjoint step of that pair and
+// of either configuration on both.  The binning of each candidate is built
 // before its timed passes: it is paid once per node set.  ms_out: forward
 // times of the configurations, then adjoint times (-1: not timed).
 template <typename T>
@@ -511,13 +513,14 @@ static void autotune(typename Traits<T>::plan *p, Impl<T> *im, int dirs, int rep
     constexpr int NC = kTileVariants + 1;
     if (dirs & ~3 || !dirs) throw Error(CUNFFT_ERR_INVALID_ARG, "directions must be 1, 2 or 3");
     if (reps <= 0) reps = 5;
-    void *fh = nullptr, *f = nullptr;
+    void *fh = nullptr, *f = nullptr, *fh2 = nullptr;   // fh2: the adjoint output of a step
     float ms[2][NC];
     int best[2] = {im->config[0], im->config[1]};
     auto finish = [&] {
         (void)hipStreamSynchronize(s);
         if (fh) (void)hipFree(fh);
         if (f) (void)hipFree(f);
+        if (fh2) (void)hipFree(fh2);
         plan_configs<T>(im, best[0], best[1]);
     };
     try {
@@ -561,6 +564,45 @@ static void autotune(typename Traits<T>::plan *p, Impl<T> *im, int dirs, int rep
                     best[dir] = c;
                 }
             }
+        }
+        // Both directions tuned to different configurations: a forward and an
+        // adjoint on one configuration share one binning (one build per node set,
+        // the same node data read by both passes), so the step of each
+        // direction's best on both directions is timed beside the step of the
+        // pair, and the fastest step is kept.
+        if (dirs == 3 && best[0] != best[1]) {
+            CUNFFT_HIP(hipMalloc(&fh2, sizeof(cplx_t<T>) * p->N_total));
+            const int pairs[3][2] = {{best[0], best[1]}, {best[1], best[1]}, {best[0], best[0]}};
+            int pick = 0;
+            float pick_ms = -1.0f;
+            for (int k = 0; k < 3; ++k) {
+                const int cf = pairs[k][0], ca = pairs[k][1];
+                if (ms[0][cf] < 0 || ms[1][ca] < 0 || !plan_configs<T>(im, cf, ca)) continue;
+                double a, b, e;
+                auto step = [&] {
+                    run_forward<T>(p, im, static_cast<const cplx_t<T> *>(fh),
+                                   static_cast<cplx_t<T> *>(f), s, false, &a, &b);
+                    run_adjoint<T>(p, im, static_cast<const cplx_t<T> *>(f),
+                                   static_cast<cplx_t<T> *>(fh2), s, false, &a, &b, &e);
+                };
+                step();
+                float t_min = 0;
+                for (int r = 0; r < reps; ++r) {
+                    CUNFFT_HIP(hipEventRecord(im->ev[0], s));
+                    step();
+                    CUNFFT_HIP(hipEventRecord(im->ev[1], s));
+                    CUNFFT_HIP(hipEventSynchronize(im->ev[1]));
+                    float t = 0;
+                    CUNFFT_HIP(hipEventElapsedTime(&t, im->ev[0], im->ev[1]));
+                    t_min = r == 0 ? t : std::min(t_min, t);
+                }
+                if (pick_ms < 0 || t_min < pick_ms) {
+                    pick_ms = t_min;
+                    pick = k;
+                }
+            }
+            best[0] = pairs[pick][0];
+            best[1] = pairs[pick][1];
         }
         CUNFFT_HIP(hipStreamSynchronize(s));
     } catch (...) {

@@ -299,7 +299,10 @@ int cunfft_solve_dev(cunfft_plan *plan, const cunfftDoubleComplex *y_dev, const 
  * nodes (set them first) in each direction of `directions`
  * (CUNFFT_TUNE_FORWARD and/or CUNFFT_TUNE_ADJOINT), best of `reps` (<= 0: 5),
  * and keeps the fastest per direction (the two directions may differ; each
- * then has its own node binning).  ms_out (may be NULL) receives
+ * then has its own node binning).  With both directions tuned and their
+ * fastest configurations different, it also times a forward + adjoint step
+ * with each of the two on both directions (one shared binning) and keeps
+ * whichever of the three steps is fastest.  ms_out (may be NULL) receives
  * 2 * CUNFFT_TUNE_CONFIGS times in milliseconds, the forward times of the
  * configurations followed by the adjoint ones, -1 where a configuration does
  * not apply or a direction was not tuned.  Allocates and frees temporaries,

@@ -4,8 +4,9 @@ cunfft_set_config).
 Every configuration -- the three tile shapes of the plane-tiled kernels and
 the node-lane kernels -- must give the oracle's NFFT (FP64 to 1e-12, FP32 to
 the FP32 bound on FP32-rounded inputs), and the autotuner must keep the
-configuration it timed fastest, leaving a plan that still transforms
-correctly.  Node sets: uniform, and clustered (N(0, 0.05^2) wrapped), which
+configuration it timed fastest per direction -- or, when the two directions'
+fastest differ, one of them on both directions (a step sharing one binning)
+-- leaving a plan that still transforms correctly.  Node sets: uniform, and clustered (N(0, 0.05^2) wrapped), which
 piles many chunks onto a few tiles.
 """
 import numpy as np
@@ -93,10 +94,14 @@ def test_autotune_keeps_the_fastest_configuration(gpu, orc, precision):
         chosen, ms_f, ms_a = p.autotune(reps=3, stream=torch.cuda.current_stream().cuda_stream)
         print(precision, "autotune", chosen, ms_f, ms_a)
         assert len(ms_f) == len(ms_a) == gpu.TUNE_CONFIGS
+        fastest = []
         for d, ms in enumerate((ms_f, ms_a)):
             valid = [c for c in range(gpu.TUNE_CONFIGS) if ms[c] >= 0]
             assert 0 in valid and gpu.TILE_SHAPES in valid
-            assert chosen[d] in valid and ms[chosen[d]] == min(ms[c] for c in valid)
+            assert chosen[d] in valid
+            fastest.append(min(valid, key=lambda c: ms[c]))
+        b0, b1 = fastest
+        assert tuple(chosen) in {(b0, b1), (b1, b1), (b0, b0)}
         assert p.config == chosen
         f, h = fwd_adj(p, x, fh, fv)
     tol = 1e-12 if precision == "double" else 1e-5

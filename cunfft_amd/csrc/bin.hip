@@ -777,8 +777,8 @@ int64_t tile_fold_blocks(int d, int variant, int64_t ntiles, int elem_bytes)
 {
     int64_t T[3];
     tile_shape(d, variant, T);
-    if (d >= 2 && elem_bytes == 4) {   // column-segment fold: 64 bytes of a core column per thread
-        const int64_t tpt = T[0] * (d == 3 ? T[1] : 1) * (T[d - 1] * 2 * elem_bytes / 64);   // threads per tile
+    if (d >= 2 && elem_bytes == 4) {   // column-segment fold: kFoldSegBytes of a core column per thread
+        const int64_t tpt = T[0] * (d == 3 ? T[1] : 1) * (T[d - 1] * 2 * elem_bytes / kFoldSegBytes);
         return tpt <= 256 ? (ntiles + 256 / tpt - 1) / (256 / tpt) : ntiles * (tpt / 256);
     }
     const int64_t core = T[0] * T[1] * T[2];

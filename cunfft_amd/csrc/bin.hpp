@@ -5,6 +5,7 @@
 namespace cunfft {
 
 constexpr int kChunk = 256;   // nodes per chunk (one workgroup's share of a tile)
+constexpr int kFoldSegBytes = 32;   // bytes of a core column one fold thread sums (FP32, d >= 2)
 
 // Grid tile per dimension: the nodes whose window starts in one tile are
 // gridded by one workgroup against the tile plus a (2m+1)-cell halo in LDS.

@@ -28,7 +28,9 @@ constexpr int kFftThreads = 256;
 // registers.
 template <typename T> constexpr int fft_max_elems() { return 4096; }
 constexpr int kFftSmallElems = 4096;
-// global loads in flight per thread (FP64: fewer, the registers run short)
+// global loads in flight per thread from fhat (FP64: fewer, the registers run
+// short); loads from the grid, with no FP64 roll-off factor to hold, take twice
+// as many (FP64 3D 256^3 passes 3% faster, FP32 unchanged)
 template <typename T> constexpr int fft_load_batch() { return sizeof(T) == 4 ? 8 : 4; }
 constexpr int kMaxRowsPerGroup = 64;
 
@@ -301,9 +303,9 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
     // first LDS store (a pass is bound by memory latency otherwise): every
     // lane loads, from a clamped address where it has nothing to load, and
     // the value is zeroed after
-    constexpr int kFftLoadBatch = fft_load_batch<T>();
     auto stage_in = [&](auto from_fhat_tag) {
         constexpr bool FROM_FHAT = decltype(from_fhat_tag)::value;
+        constexpr int kFftLoadBatch = fft_load_batch<T>() * (FROM_FHAT ? 1 : 2);
         const bool in_pruned = P.mode & FFT_IN_PRUNED;
         for (int e0 = tid; e0 < total; e0 += kFftLoadBatch * kFftThreads) {
             C2 v[kFftLoadBatch];

@@ -425,7 +425,8 @@ class Plan:
         """Time the gridding configurations on the plan's current nodes and keep
         the fastest per direction, or, when both directions are tuned to
         different ones, the fastest forward + adjoint step of that pair and of
-        either one on both directions (cunfft_autotune).  Returns ((forward config,
+        either one on both directions, the pair only if its step is more than 2%
+        faster (cunfft_autotune).  Returns ((forward config,
         adjoint config), forward times, adjoint times) with times in ms per
         configuration, -1 where a configuration does not apply or a direction
         was not tuned."""

@@ -302,7 +302,10 @@ int cunfft_solve_dev(cunfft_plan *plan, const cunfftDoubleComplex *y_dev, const 
  * then has its own node binning).  With both directions tuned and their
  * fastest configurations different, it also times a forward + adjoint step
  * with each of the two on both directions (one shared binning) and keeps
- * whichever of the three steps is fastest.  ms_out (may be NULL) receives
+ * whichever of the three steps is fastest; the pair of different
+ * configurations only if its step is more than 2% faster than the shared
+ * one's (a shared binning takes half the memory and one sort per node set).
+ * ms_out (may be NULL) receives
  * 2 * CUNFFT_TUNE_CONFIGS times in milliseconds, the forward times of the
  * configurations followed by the adjoint ones, -1 where a configuration does
  * not apply or a direction was not tuned.  Allocates and frees temporaries,

@@ -1424,6 +1424,9 @@ __device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slo
             constexpr int QL = L::QL;
             const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1);
             const bool in01 = r0 < E0 && r1 < E1;
+            // a halo block pair that covers none of the wavefront's columns
+            // (E0 = 1 past the first rows of the tile) is not loaded at all
+            if (!__any(in01)) continue;
             const int rc = ((r0 < E0 ? r0 : 0) * E1 + (r1 < E1 ? r1 : 0));
             uint32_t c0[QL], nch[QL];
             bool ok[QL];

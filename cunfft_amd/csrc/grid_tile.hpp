@@ -1035,8 +1035,10 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
     uint32_t ad[NI][OWN];
     int cur_key = -1;
     static_assert(L::T0 + NW - 2 < 64, "a wavefront's first plane fits the key's low 6 bits");
+    // the batch's node count in a scalar register: the node loop's exit test is
+    // then a scalar compare (left alone, the clamp is a vector med3)
     for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
-        const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
+        const int nb = __builtin_amdgcn_readfirstlane(ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB);
         if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch
         const NodeRec rc = batch_prologue_wg<T, D, W, NB, L, NW, NW>(wb, nb, lane, wave, nd, fv, wc);
         {

@@ -867,7 +867,8 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
     WinBatch<T, D, W, NB> &wb = wbs[wave];
 
     for (uint32_t s0 = 0; s0 < ch.count; s0 += NB) {
-        const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
+        // scalar, so the node loop exits on a scalar compare
+        const int nb = __builtin_amdgcn_readfirstlane(ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB);
         const NodeRec rc = batch_prologue<T, D, W, NB, L>(wb, nb, lane, nd, wc);
         const C fc = fv;
         {

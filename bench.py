@@ -278,6 +278,50 @@ def cpu_baseline(x, fh, budget_s=15.0):
             "cpu_model": model, "nproc": os.cpu_count()}
 
 
+BATCH_P, BATCH_R, BATCH_M = 96, 8, 4
+
+
+def batch_workload(seed=5150):
+    """The batched workload: 96 2D problems N_b x N_b, N_b drawn from
+    {64, 96, 128, 192, 256}, sigma = 2 exactly (n_b = 2 N_b), m = 4, M_b
+    uniform in [4096, 65536], 8 right-hand sides.  Returns (N, n, M,
+    problem) with problem(i) -> (x (M_i, 2), fhat_in (8, N_i^2), f_in (8, M_i)),
+    the coefficients complex64 with re/im N(0, 1); each problem has its own
+    seeded stream, so any one can be rebuilt alone."""
+    r = np.random.default_rng(seed)
+    sizes = r.choice([64, 96, 128, 192, 256], BATCH_P)
+    N = np.stack([sizes, sizes], axis=1)
+    M = r.integers(4096, 65536, BATCH_P, endpoint=True)
+
+    def problem(i):
+        ri = np.random.default_rng([seed, i])
+        Mi, Nt = int(M[i]), int(N[i][0] * N[i][1])
+        x = ri.uniform(-0.5, 0.5, (Mi, 2))
+        fh = (ri.standard_normal((BATCH_R, Nt)) + 1j * ri.standard_normal((BATCH_R, Nt))).astype(np.complex64)
+        fv = (ri.standard_normal((BATCH_R, Mi)) + 1j * ri.standard_normal((BATCH_R, Mi))).astype(np.complex64)
+        return x, fh, fv
+
+    return N, 2 * N, M, problem
+
+
+def load_batch(b, problem, dev):
+    """Nodes and inputs of every problem into the batch b; returns the device
+    node arrays (kept alive by the caller)."""
+    import torch
+
+    xs = []
+    for i in range(b.P):
+        x, fh, fv = problem(i)
+        xt = torch.from_numpy(x).to(dev)
+        xs.append(xt)
+        b.set_nodes_dev(i, xt.data_ptr())
+        fhat_in, _, f_in, _ = b.tensors(i)
+        fhat_in.copy_(torch.from_numpy(fh.reshape(-1)).to(dev))
+        f_in.copy_(torch.from_numpy(fv.reshape(-1)).to(dev))
+    torch.cuda.synchronize()
+    return xs
+
+
 def bench_batch(cu, steps, warmup, seed=5150):
     """96 uneven 2D FP32 problems x 8 right-hand sides: one graph launch of the
     batched stage kernels per step, against a loop of single-plan launches."""
@@ -285,24 +329,12 @@ def bench_batch(cu, steps, warmup, seed=5150):
 
     from cunfft_amd.batch import Batch
 
-    r = np.random.default_rng(seed)
-    P, R, m = 96, 8, 4
-    sizes = r.choice([64, 96, 128, 192, 256], P)
-    N = np.stack([sizes, sizes], axis=1)
-    M = r.integers(4096, 65536, P, endpoint=True)
+    P, R, m = BATCH_P, BATCH_R, BATCH_M
+    N, n, M, problem = batch_workload(seed)
+    sizes = N[:, 0]
     dev = torch.device("cuda")
-    b = Batch(N, M, m=m, n_rhs=R, n=2 * N, precision="single")   # sigma = 2 exactly
-    xs = []
-    for i in range(P):
-        x = torch.from_numpy(r.uniform(-0.5, 0.5, (int(M[i]), 2))).to(dev)
-        xs.append(x)
-        b.set_nodes_dev(i, x.data_ptr())
-        fhat_in, _, f_in, _ = b.tensors(i)
-        fhat_in.copy_(torch.complex(torch.randn(fhat_in.shape, device=dev),
-                                    torch.randn(fhat_in.shape, device=dev)))
-        f_in.copy_(torch.complex(torch.randn(f_in.shape, device=dev),
-                                 torch.randn(f_in.shape, device=dev)))
-    torch.cuda.synchronize()
+    b = Batch(N, M, m=m, n_rhs=R, n=n, precision="single")   # sigma = 2 exactly
+    xs = load_batch(b, problem, dev)
     config, ms_f, ms_a = b.autotune(reps=3)
     for _ in range(warmup):
         b.execute()

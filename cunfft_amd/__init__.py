@@ -39,6 +39,7 @@ NODE_LANES = 1 << 12
 COMPENSATED = 1 << 13
 ASYNC_COPY = 1 << 14
 KAISER_BESSEL = 1 << 15
+SINC = 1 << 16
 DEFAULT_CUT_OFF = 6
 TILE_SHAPES = 3                     # plane-tiled configurations 0 .. TILE_SHAPES-1
 TUNE_CONFIGS = TILE_SHAPES + 1      # + the node-lane configuration

@@ -148,7 +148,8 @@ template <typename T> struct BatchImpl {
                 nn[t] = n ? n[(size_t)i * d + t] : 2 * v;   // src/cunfft.cu:296-299
                 if (NN[t] < 2 || (NN[t] & 1))
                     throw Error(CUNFFT_ERR_INVALID_ARG, "N_t must be even and >= 2");
-                if (nn[t] < NN[t] || !fft_length_ok(nn[t]))
+                // batches run every axis as one pass (no split axes)
+                if (nn[t] < NN[t] || !fft_line_ok(nn[t]))
                     throw Error(CUNFFT_ERR_UNSUPPORTED,
                                 "n_t must be 2^a 3^b (a >= 1) in [N_t, 4096]");
                 if (nn[t] < W) throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= 2m+2");
@@ -413,10 +414,62 @@ template <typename T> struct BatchImpl {
         CUNFFT_HIP(hipGraphLaunch(graph(dirs), s));
     }
 
+    // The inputs the graphs read (fhat_in, f_in of every problem) filled with
+    // nonzero data for timing, and put back after: the node-lane scatter skips
+    // its global atomics for zero partial sums, so timing zeroed buffers (as
+    // create() leaves them) would flatter it.  Returns the saved copies.
+    std::vector<void *> fill_inputs(hipStream_t s)
+    {
+        std::vector<void *> saved;
+        try {
+            for (Prob &p : probs) {
+                const size_t bh = sizeof(C) * R * p.N_total, bf = sizeof(C) * R * p.M;
+                saved.push_back(alloc(bh));
+                CUNFFT_HIP(hipMemcpyAsync(saved.back(), p.fhat_in, bh, hipMemcpyDeviceToDevice, s));
+                saved.push_back(alloc(bf));
+                CUNFFT_HIP(hipMemcpyAsync(saved.back(), p.f_in, bf, hipMemcpyDeviceToDevice, s));
+                // words 0x3ff00000: 1.875f, or 1.0000002 as a double
+                CUNFFT_HIP(hipMemsetD32Async((hipDeviceptr_t)p.fhat_in, 0x3ff00000, bh / 4, s));
+                CUNFFT_HIP(hipMemsetD32Async((hipDeviceptr_t)p.f_in, 0x3ff00000, bf / 4, s));
+            }
+        } catch (...) {
+            restore_inputs(saved, s);
+            throw;
+        }
+        return saved;
+    }
+
+    void restore_inputs(std::vector<void *> &saved, hipStream_t s)
+    {
+        for (size_t i = 0; i < saved.size(); ++i) {
+            Prob &p = probs[i / 2];
+            const bool fh = i % 2 == 0;
+            (void)hipMemcpyAsync(fh ? (void *)p.fhat_in : (void *)p.f_in, saved[i],
+                                 sizeof(C) * R * (fh ? p.N_total : p.M), hipMemcpyDeviceToDevice, s);
+        }
+        (void)hipStreamSynchronize(s);
+        for (void *q : saved) (void)hipFree(q);
+        saved.clear();
+    }
+
     // Times each configuration (both directions in one configuration, the
-    // batch's graphs of one direction each) and keeps the fastest per tuned
-    // direction.  ms_out: forward times, then adjoint times (-1: not timed).
+    // batch's graphs of one direction each) on nonzero inputs and keeps the
+    // fastest per tuned direction.  ms_out: forward times, then adjoint times
+    // (-1: not timed).  The batch's inputs are unchanged afterwards; its
+    // outputs hold the last timed configuration's results.
     void autotune(int dirs, int reps, float *ms_out, hipStream_t s)
+    {
+        std::vector<void *> saved = fill_inputs(s);
+        try {
+            autotune_timed(dirs, reps, ms_out, s);
+        } catch (...) {
+            restore_inputs(saved, s);
+            throw;
+        }
+        restore_inputs(saved, s);
+    }
+
+    void autotune_timed(int dirs, int reps, float *ms_out, hipStream_t s)
     {
         constexpr int NC = kTileVariants + 1;
         if (reps <= 0) reps = 5;

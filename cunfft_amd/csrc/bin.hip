@@ -19,6 +19,8 @@
 // of the node-parallel kernels, so both paths compute the same taps.
 #include <hip/amd_detail/amd_hip_unsafe_atomics.h>
 
+#include <algorithm>
+
 #include "bin.hpp"
 
 namespace cunfft {
@@ -60,37 +62,39 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, int64_t n, uint32
     if (i < n) p[i] = v;
 }
 
-// Tile key of every node, its count, and (plane-tiled kernels) its window
-// descriptor by node index: the FP64 index/fraction split of window_start,
-// lo_t the window start relative to the origin of the node's tile, in
-// [0, T_t) by construction of the key.
+// Composite sort key of every node (its tile times sub_range plus, for the
+// plane-tiled kernels, its sub_key within the tile; sub_range 1 otherwise),
+// its tile's count, and (plane-tiled kernels) its window descriptor by node
+// index: the FP64 index/fraction split of window_cell, lo_t the window start
+// relative to the origin of the node's tile, in [0, T_t) by construction of
+// the key.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
                                                   uint32_t *__restrict__ keys,
                                                   NodeDesc<T> *__restrict__ desc_j,
                                                   uint32_t *__restrict__ tile_count, Geom ge,
-                                                  BinGeom bg)
+                                                  BinGeom bg, uint32_t sub_range)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= ge.M) return;
-    uint32_t key = 0, loc = 0;
+    uint32_t tile = 0, loc = 0;
+    uint32_t lo[3] = {0, 0, 0};
     NodeDesc<T> nd{};
     for (int t = 0; t < bg.d; ++t) {
-        int64_t u;
         double c;
-        window_start(x[j * bg.d + t], ge.n[t], ge.m, u, c);
-        const int64_t w = wrap_lo(u, ge.n[t]);
+        const int64_t w = window_cell(x[j * bg.d + t], ge.n[t], ge.m, c);
         const uint32_t tc = (uint32_t)(w >> bg.logT[t]);
-        loc |= (uint32_t)(w - ((int64_t)tc << bg.logT[t])) << (8 * t);
+        lo[t] = (uint32_t)(w - ((int64_t)tc << bg.logT[t]));
+        loc |= lo[t] << (8 * t);
         nd.f[t] = (T)(c - (double)ge.m);
-        key = key * (uint32_t)bg.nt[t] + tc;
+        tile = tile * (uint32_t)bg.nt[t] + tc;
     }
-    keys[j] = key;
+    keys[j] = sub_range > 1 ? tile * sub_range + sub_key(bg.d, lo, bg.logT, bg.sub_shift) : tile;
     if (desc_j) {
         nd.loc = loc;
         desc_j[j] = nd;
     }
-    atomicAdd(&tile_count[key], 1u);
+    atomicAdd(&tile_count[tile], 1u);
 }
 
 // Exclusive scans of the tile counts and of the chunk counts, in three
@@ -136,6 +140,8 @@ __device__ __forceinline__ void block_scan2(uint32_t &a, uint32_t &b, uint32_t *
     }
 }
 
+// (a, b) = (count, chunks of count) of element i of cnt, or of the digit
+// counts when chunks is false (b then unused).
 __global__ __launch_bounds__(kScanBlock) void k_bin_scan_part(const uint32_t *__restrict__ cnt,
                                                               uint32_t *__restrict__ part,
                                                               int64_t ntiles, int64_t nparts)
@@ -151,7 +157,8 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan_part(const uint32_t *__
     }
 }
 
-// One workgroup: exclusive scan of the block sums in place; the chunk total.
+// One workgroup: exclusive scan of the block sums in place; the chunk total
+// (n_chunks may be NULL).
 __global__ __launch_bounds__(kScanBlock) void k_bin_scan_top(uint32_t *__restrict__ part,
                                                              int64_t nparts,
                                                              uint32_t *__restrict__ n_chunks)
@@ -172,13 +179,14 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan_top(uint32_t *__restric
         cb += wb[kScanBlock / 64 - 1];
         __syncthreads();
     }
-    if (threadIdx.x == 0) n_chunks[0] = cb;
+    if (threadIdx.x == 0 && n_chunks) n_chunks[0] = cb;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_bin_scan_apply(const uint32_t *__restrict__ cnt,
+// start = exclusive scan of cnt; chunk_start (may be NULL) = exclusive scan
+// of the chunk counts.  start may be cnt itself (in place).
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan_apply(const uint32_t *cnt,
                                                                const uint32_t *__restrict__ part,
-                                                               uint32_t *__restrict__ start,
-                                                               uint32_t *__restrict__ cursor,
+                                                               uint32_t *start,
                                                                uint32_t *__restrict__ chunk_start,
                                                                int64_t ntiles, int64_t nparts)
 {
@@ -189,89 +197,114 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan_apply(const uint32_t *_
     block_scan2(a, b, wa, wb);
     if (i < ntiles) {
         start[i] = part[blockIdx.x] + a - c;
-        chunk_start[i] = part[nparts + blockIdx.x] + b - cc;
-        cursor[i] = 0;
+        if (chunk_start) chunk_start[i] = part[nparts + blockIdx.x] + b - cc;
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_bin_place(const double *__restrict__ x,
-                                                   const uint32_t *__restrict__ keys,
-                                                   const uint32_t *__restrict__ start,
-                                                   uint32_t *__restrict__ cursor,
-                                                   uint32_t *__restrict__ perm,
-                                                   double *__restrict__ xs,
-                                                   const NodeDesc<T> *__restrict__ desc_j,
-                                                   NodeDesc<T> *__restrict__ desc_out, int64_t M, int d)
+// ------------------------------------------------------- radix sort ----
+// Stable LSD radix sort of (composite key, node index) pairs, 8-bit digits.
+// Each pass: per-block digit counts (k_rs_hist), an exclusive scan of the
+// counts in digit-major order (digit d of block b at d * nblk + b), then a
+// stable scatter (k_rs_scatter) in which block b walks its items in order,
+// 256 at a time, and ranks each among the earlier items of its digit: within
+// a wavefront from ballots on the digit's bits, across the block's wavefronts
+// and rounds from LDS counters.  No atomic decides a position, so the result
+// is the same on every run.
+constexpr int kRsThreads = 256;
+constexpr int kRsRounds = 16;
+constexpr int kRsItems = kRsThreads * kRsRounds;   // items per block
+constexpr int kRsDigits = 256;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t M,
+                                                        int shift, uint32_t *__restrict__ hist,
+                                                        uint32_t nblk)
 {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= M) return;
-    const uint32_t k = keys[j];
-    const uint32_t pos = start[k] + atomicAdd(&cursor[k], 1u);
-    perm[pos] = (uint32_t)j;
-    if (desc_j) desc_out[pos] = desc_j[j];
-    if (xs)
-        for (int t = 0; t < d; ++t) xs[(int64_t)pos * d + t] = x[j * d + t];
+    __shared__ uint32_t h[kRsDigits];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i0 = (int64_t)blockIdx.x * kRsItems + threadIdx.x;
+#pragma unroll 4
+    for (int k = 0; k < kRsRounds; ++k) {
+        const int64_t i = i0 + (int64_t)k * kRsThreads;
+        if (i < M) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Second pass of the sort, one wavefront per tile: a counting sort of the
-// tile's nodes on sub_key of their window start (LDS histogram of nrange
-// bins, wavefront scan, placement), from the placement order (perm_in,
-// desc_in) to the final one (perm_out, desc_out).  Equal keys keep no
-// particular order, as in the placement.  nrange is a power of two >= 64.
-template <typename T>
-__device__ __forceinline__ uint32_t desc_sub_key(const NodeDesc<T> &nd, const BinGeom &bg)
+// vals_in NULL: the values are the item indices (the first pass over node
+// order).  keys_out NULL: the sorted keys are not needed (last pass).
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t *__restrict__ keys_in,
+                                                           const uint32_t *__restrict__ vals_in,
+                                                           int64_t M, int shift,
+                                                           const uint32_t *__restrict__ offs,
+                                                           uint32_t nblk,
+                                                           uint32_t *__restrict__ keys_out,
+                                                           uint32_t *__restrict__ vals_out)
 {
-    const uint32_t loc = (uint32_t)nd.loc;
-    const uint32_t lo[3] = {loc & 255, (loc >> 8) & 255, (loc >> 16) & 255};
-    return sub_key(bg.d, lo, bg.logT, bg.sub_shift);
-}
-
-template <typename T>
-__global__ __launch_bounds__(64) void k_bin_subsort(const uint32_t *__restrict__ cnt,
-                                                    const uint32_t *__restrict__ start,
-                                                    const uint32_t *__restrict__ perm_in,
-                                                    const NodeDesc<T> *__restrict__ desc_in,
-                                                    uint32_t *__restrict__ perm_out,
-                                                    NodeDesc<T> *__restrict__ desc_out,
-                                                    uint32_t nrange, BinGeom bg)
-{
-    extern __shared__ uint32_t hist[];
-    const uint32_t tile = blockIdx.x, lane = threadIdx.x;
-    const uint32_t c = cnt[tile], s0 = start[tile];
-    if (c <= 1) {
-        if (c == 1 && lane == 0) {
-            perm_out[s0] = perm_in[s0];
-            desc_out[s0] = desc_in[s0];
+    __shared__ uint32_t base[kRsDigits];                        // next position per digit
+    __shared__ uint32_t wcnt[kRsThreads / 64][kRsDigits];       // this round's count per wavefront
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    base[tid] = offs[(size_t)tid * nblk + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kRsThreads / 64; ++w) wcnt[w][tid] = 0;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __syncthreads();
+    const int64_t i0 = (int64_t)blockIdx.x * kRsItems + tid;
+    for (int k = 0; k < kRsRounds; ++k) {
+        const int64_t i = i0 + (int64_t)k * kRsThreads;
+        if (((int64_t)blockIdx.x * kRsItems + (int64_t)k * kRsThreads) >= M) break;   // uniform
+        const bool valid = i < M;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? (vals_in ? vals_in[i] : (uint32_t)i) : 0u;
+        const uint32_t dg = (key >> shift) & 255u;
+        // lanes of this wavefront holding the same digit (valid lanes only)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint64_t ones = __ballot(valid && ((dg >> bit) & 1u));
+            peers &= ((dg >> bit) & 1u) ? ones : ~ones;
         }
-        return;
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[dg] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][dg];
+            if (keys_out) keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < kRsThreads / 64; ++w) {
+            tot += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        base[tid] += tot;
+        __syncthreads();
     }
-    for (uint32_t i = lane; i < nrange; i += 64) hist[i] = 0;
-    __syncthreads();
-    for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[desc_sub_key(desc_in[s0 + i], bg)], 1u);
-    __syncthreads();
-    // exclusive scan: lane owns bins [lane * per, (lane + 1) * per)
-    const uint32_t per = nrange / 64, b0 = lane * per;
-    uint32_t sum = 0;
-    for (uint32_t i = 0; i < per; ++i) sum += hist[b0 + i];
-    uint32_t inc = sum;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = __shfl_up(inc, off, 64);
-        if ((int)lane >= off) inc += v;
-    }
-    uint32_t run = inc - sum;
-    for (uint32_t i = 0; i < per; ++i) {
-        const uint32_t h = hist[b0 + i];
-        hist[b0 + i] = run;
-        run += h;
-    }
-    __syncthreads();
-    for (uint32_t i = lane; i < c; i += 64) {
-        const NodeDesc<T> nd = desc_in[s0 + i];
-        const uint32_t pos = s0 + atomicAdd(&hist[desc_sub_key(nd, bg)], 1u);
-        perm_out[pos] = perm_in[s0 + i];
-        desc_out[pos] = nd;
-    }
+}
+
+// Sorted-order copies of the per-node data: descriptors (plane-tiled
+// kernels) or coordinates (node-lane kernels).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_gather_desc(const uint32_t *__restrict__ perm,
+                                                         const NodeDesc<T> *__restrict__ desc_j,
+                                                         NodeDesc<T> *__restrict__ desc, int64_t M)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s < M) desc[s] = desc_j[perm[s]];
+}
+
+__global__ __launch_bounds__(256) void k_bin_gather_x(const uint32_t *__restrict__ perm,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ xs, int64_t M, int d)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const int64_t j = perm[s];
+    for (int t = 0; t < d; ++t) xs[s * d + t] = x[j * d + t];
 }
 
 __global__ __launch_bounds__(256) void k_bin_chunks(const uint32_t *__restrict__ cnt,
@@ -328,73 +361,163 @@ bool Binning::plan(const Geom &ge, int elem_bytes_, int kernel_variant, int tile
         if (bytes > kMaxTileBytes) return false;
         if (d == 3 && W > 14) return false;
     }
+    // the composite sort key must fit 32 bits
+    const uint64_t sr = planes() ? sub_range(bin_geom(*this)) : 1u;
+    const uint64_t kmax = (uint64_t)ntiles * sr;
+    if (kmax > (uint64_t(1) << 32)) return false;
+    int bits = 1;
+    while ((uint64_t(1) << bits) < kmax) ++bits;
+    sort_passes = (bits + 7) / 8;
     usable = 1;
     return true;
 }
 
-static void *dev_alloc(size_t bytes)
+static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Offsets of the buffers in the binning's device block (carve == true: set
+// the pointers from base).  Sizes:
+//   keys, keys2, perm, perm2    M u32 each
+//   rs_hist                     256 x radix blocks u32
+//   scan_part                   2 x the blocks of the larger scan
+//   tile_count/_start, chunk_start  ntiles u32 each; n_chunks 1
+//   chunks                      max_chunks
+//   plane-tiled: desc, desc2    M descriptors each; slots, fs (own scratch)
+//   node-lane:   xs             M d doubles
+static size_t bin_layout(Binning &b, int64_t M, char *base)
 {
-    void *p = nullptr;
-    CUNFFT_HIP(hipMalloc(&p, bytes ? bytes : 16));
-    return p;
+    const int64_t nblk = (M + kRsItems - 1) / kRsItems;
+    const int64_t max_chunks = b.ntiles + (M + kChunk - 1) / kChunk;
+    const int64_t scan_n = std::max<int64_t>(b.ntiles, kRsDigits * nblk);
+    const size_t db = b.elem_bytes == 4 ? sizeof(NodeDesc<float>) : sizeof(NodeDesc<double>);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes ? bytes : 16);
+        return p;
+    };
+    uint32_t *keys = (uint32_t *)take(4 * (size_t)M), *keys2 = (uint32_t *)take(4 * (size_t)M);
+    uint32_t *perm = (uint32_t *)take(4 * (size_t)M), *perm2 = (uint32_t *)take(4 * (size_t)M);
+    uint32_t *rs_hist = (uint32_t *)take(4 * (size_t)kRsDigits * nblk);
+    uint32_t *scan_part = (uint32_t *)take(4 * 2 * (size_t)((scan_n + kScanBlock - 1) / kScanBlock));
+    uint32_t *tile_count = (uint32_t *)take(4 * (size_t)b.ntiles);
+    uint32_t *tile_start = (uint32_t *)take(4 * (size_t)b.ntiles);
+    uint32_t *chunk_start = (uint32_t *)take(4 * (size_t)b.ntiles);
+    uint32_t *n_chunks = (uint32_t *)take(4);
+    Chunk *chunks = (Chunk *)take(sizeof(Chunk) * (size_t)max_chunks);
+    void *desc = nullptr, *desc2 = nullptr, *slots = nullptr, *fs = nullptr;
+    double *xs = nullptr;
+    if (b.planes()) {
+        desc = take((size_t)M * db);
+        desc2 = take((size_t)M * db);
+        if (b.own_scratch) {
+            slots = take((size_t)max_chunks * b.slot_cells * 2 * b.elem_bytes);
+            fs = take((size_t)M * 2 * b.elem_bytes);
+        }
+    } else {
+        xs = (double *)take(sizeof(double) * (size_t)M * b.d);
+    }
+    if (base) {
+        b.keys = keys;
+        b.keys2 = keys2;
+        b.perm = perm;
+        b.perm2 = perm2;
+        b.rs_hist = rs_hist;
+        b.scan_part = scan_part;
+        b.tile_count = tile_count;
+        b.tile_start = tile_start;
+        b.chunk_start = chunk_start;
+        b.n_chunks = n_chunks;
+        b.chunks = chunks;
+        b.desc = desc;
+        b.desc2 = desc2;
+        b.slots = slots;
+        b.fs = fs;
+        b.xs = xs;
+        b.max_chunks = max_chunks;
+        b.M = M;
+    }
+    return off;
+}
+
+size_t Binning::bytes_for(int64_t M_) const { return bin_layout(const_cast<Binning &>(*this), M_, nullptr); }
+
+void Binning::carve(char *base, int64_t M_) { bin_layout(*this, M_, base); }
+
+bool Binning::bind(void *base, size_t bytes, int64_t M_)
+{
+    if (!usable || bytes < bytes_for(M_)) return false;
+    release();
+    block = base;
+    own_block = false;
+    carve(static_cast<char *>(base), M_);
+    return true;
+}
+
+// Scan of n counts cnt in place into their exclusive prefix (start == cnt
+// allowed) and, for tile counts, of their chunk counts into chunk_start.
+static void scan_counts(const uint32_t *cnt, uint32_t *start, uint32_t *chunk_start,
+                        uint32_t *n_chunks, uint32_t *part, int64_t n, hipStream_t s)
+{
+    const int64_t nparts = (n + kScanBlock - 1) / kScanBlock;
+    k_bin_scan_part<<<(unsigned)nparts, kScanBlock, 0, s>>>(cnt, part, n, nparts);
+    k_bin_scan_top<<<1, kScanBlock, 0, s>>>(part, nparts, n_chunks);
+    k_bin_scan_apply<<<(unsigned)nparts, kScanBlock, 0, s>>>(cnt, part, start, chunk_start, n, nparts);
 }
 
 // The binning passes: keys (and, for the plane-tiled kernels, descriptors by
-// node), tile scan, placement, chunk list, and the sort within tiles.
+// node), tile scan and chunk list, the radix sort of the keys, and the
+// sorted copies of the per-node data.
 template <typename T> void Binning::run_build(const Geom &ge, const BinGeom &bg, const double *x,
                                               hipStream_t s)
 {
     const unsigned gm = (unsigned)((M + 255) / 256);
-    const int64_t nparts = (ntiles + kScanBlock - 1) / kScanBlock;
-    NodeDesc<T> *dj = planes() ? static_cast<NodeDesc<T> *>(desc) : nullptr;   // by node until placed
-    NodeDesc<T> *dp = planes() ? static_cast<NodeDesc<T> *>(desc2) : nullptr;  // placement order
+    NodeDesc<T> *dj = planes() ? static_cast<NodeDesc<T> *>(desc2) : nullptr;   // by node
+    const uint32_t sr = planes() ? sub_range(bg) : 1u;
     k_fill_u32<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, ntiles, 0u);
-    k_bin_keys<T><<<gm, 256, 0, s>>>(x, keys, dj, tile_count, ge, bg);
-    k_bin_scan_part<<<(unsigned)nparts, kScanBlock, 0, s>>>(tile_count, scan_part, ntiles, nparts);
-    k_bin_scan_top<<<1, kScanBlock, 0, s>>>(scan_part, nparts, n_chunks);
-    k_bin_scan_apply<<<(unsigned)nparts, kScanBlock, 0, s>>>(tile_count, scan_part, tile_start, cursor,
-                                                            chunk_start, ntiles, nparts);
-    k_bin_place<T><<<gm, 256, 0, s>>>(x, keys, tile_start, cursor, planes() ? perm2 : perm, xs, dj,
-                                      dp, M, d);
+    k_bin_keys<T><<<gm, 256, 0, s>>>(x, keys, dj, tile_count, ge, bg, sr);
+    scan_counts(tile_count, tile_start, chunk_start, n_chunks, scan_part, ntiles, s);
     k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
                                                                   chunk_start, chunks, ntiles);
-    if (planes()) {
-        const uint32_t nr = sub_range(bg);
-        if (nr < 64 || (nr & (nr - 1)) != 0 || nr > 4096)
-            throw Error(CUNFFT_ERR_UNSUPPORTED, "tile sub-key range must be a power of two in [64, 4096]");
-        k_bin_subsort<T><<<(unsigned)ntiles, 64, nr * sizeof(uint32_t), s>>>(
-            tile_count, tile_start, perm2, dp, perm, static_cast<NodeDesc<T> *>(desc), nr, bg);
+    // radix passes: pass p reads (kin, vin) and writes (kout, vout); the
+    // destinations alternate so that the last pass writes perm
+    const uint32_t nblk = (uint32_t)((M + kRsItems - 1) / kRsItems);
+    uint32_t *kbuf[2] = {keys, keys2};
+    uint32_t *vbuf[2] = {perm, perm2};
+    for (int p = 0; p < sort_passes; ++p) {
+        const int dst = (sort_passes - 1 - p) & 1;   // the last pass writes buffer 0 (perm)
+        const uint32_t *kin = p == 0 ? keys : kbuf[dst ^ 1];
+        const uint32_t *vin = p == 0 ? nullptr : vbuf[dst ^ 1];
+        // the first pass reads keys (buffer 0); it must not also write keys
+        uint32_t *kout = p + 1 < sort_passes ? (p == 0 ? keys2 : kbuf[dst]) : nullptr;
+        uint32_t *vout = vbuf[dst];
+        k_rs_hist<<<nblk, kRsThreads, 0, s>>>(kin, M, 8 * p, rs_hist, nblk);
+        scan_counts(rs_hist, rs_hist, nullptr, nullptr, scan_part, (int64_t)kRsDigits * nblk, s);
+        k_rs_scatter<<<nblk, kRsThreads, 0, s>>>(kin, vin, M, 8 * p, rs_hist, nblk, kout, vout);
+        if (p == 0) {
+            // later passes ping-pong between keys2 and keys
+            kbuf[dst] = keys2;
+            kbuf[dst ^ 1] = keys;
+        }
     }
+    if (planes())
+        k_bin_gather_desc<T><<<gm, 256, 0, s>>>(perm, dj, static_cast<NodeDesc<T> *>(desc), M);
+    else
+        k_bin_gather_x<<<gm, 256, 0, s>>>(perm, x, xs, M, d);
 }
 
 void Binning::build(const Geom &ge, const double *x, hipStream_t s)
 {
     if (!usable) return;
-    if (!built || M != ge.M) {
+    if (!block || M != ge.M) {
+        if (block && !own_block)
+            throw Error(CUNFFT_ERR_STATE, "bound binning memory is sized for another node count");
         release();
-        M = ge.M;
-        max_chunks = ntiles + (M + kChunk - 1) / kChunk;
-        keys = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-        perm = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-        if (planes()) perm2 = (uint32_t *)dev_alloc(sizeof(uint32_t) * M);
-        scan_part = (uint32_t *)dev_alloc(sizeof(uint32_t) * 2 * ((ntiles + kScanBlock - 1) / kScanBlock));
-        tile_count = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
-        tile_start = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
-        cursor = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
-        chunk_start = (uint32_t *)dev_alloc(sizeof(uint32_t) * ntiles);
-        n_chunks = (uint32_t *)dev_alloc(sizeof(uint32_t));
-        chunks = (Chunk *)dev_alloc(sizeof(Chunk) * max_chunks);
-        if (planes()) {
-            const size_t db = elem_bytes == 4 ? sizeof(NodeDesc<float>) : sizeof(NodeDesc<double>);
-            desc = dev_alloc((size_t)M * db);
-            desc2 = dev_alloc((size_t)M * db);
-            if (own_scratch) {
-                slots = dev_alloc((size_t)max_chunks * slot_cells * 2 * elem_bytes);
-                fs = dev_alloc((size_t)M * 2 * elem_bytes);
-            }
-        } else {
-            xs = (double *)dev_alloc(sizeof(double) * M * d);
-        }
+        const size_t bytes = bytes_for(ge.M);
+        void *p = nullptr;
+        CUNFFT_HIP(hipMalloc(&p, bytes));
+        block = p;
+        own_block = true;
+        carve(static_cast<char *>(p), ge.M);
     }
     const BinGeom bg = bin_geom(*this);
     if (elem_bytes == 4) run_build<float>(ge, bg, x, s);
@@ -405,28 +528,14 @@ void Binning::build(const Geom &ge, const double *x, hipStream_t s)
 
 void Binning::release()
 {
-    if (keys) (void)hipFree(keys);
-    if (scan_part) (void)hipFree(scan_part);
-    if (desc2) (void)hipFree(desc2);
-    if (perm) (void)hipFree(perm);
-    if (perm2) (void)hipFree(perm2);
-    if (tile_count) (void)hipFree(tile_count);
-    if (tile_start) (void)hipFree(tile_start);
-    if (cursor) (void)hipFree(cursor);
-    if (chunk_start) (void)hipFree(chunk_start);
-    if (n_chunks) (void)hipFree(n_chunks);
-    if (chunks) (void)hipFree(chunks);
-    if (xs) (void)hipFree(xs);
-    if (desc) (void)hipFree(desc);
-    if (slots) (void)hipFree(slots);
-    if (fs) (void)hipFree(fs);
-    keys = scan_part = perm = perm2 = tile_count = tile_start = cursor = chunk_start = n_chunks = nullptr;
-    desc2 = nullptr;
+    if (block && own_block) (void)hipFree(block);
+    block = nullptr;
+    own_block = false;
+    keys = keys2 = perm = perm2 = rs_hist = scan_part = nullptr;
+    tile_count = tile_start = chunk_start = n_chunks = nullptr;
     chunks = nullptr;
     xs = nullptr;
-    desc = nullptr;
-    slots = nullptr;
-    fs = nullptr;
+    desc = desc2 = slots = fs = nullptr;
     built = 0;
     M = 0;
 }
@@ -473,10 +582,8 @@ __device__ __forceinline__ int node_local(const double *__restrict__ xs, uint32_
     int base = 0;
 #pragma unroll
     for (int t = 0; t < D; ++t) {
-        int64_t u;
         double c;
-        window_start(xs[(int64_t)s * D + t], ge.n[t], ge.m, u, c);
-        int lo = (int)(wrap_lo(u, ge.n[t]) - o[t]);
+        int lo = (int)(window_cell(xs[(int64_t)s * D + t], ge.n[t], ge.m, c) - o[t]);
         lo = lo < 0 ? 0 : (lo >= TileShape<D>::T[t] ? TileShape<D>::T[t] - 1 : lo);
         window_weights<T, W>(c, ge, t, w[t]);
         base += lo * stride[t];

@@ -59,15 +59,22 @@ template <> struct alignas(16) NodeDesc<double> {
 
 struct BinGeom;
 
-// Nodes sorted (counting sort on the device) by the grid tile that holds the
-// start of their window, u_t mod n_t.  Built once per node set on the plan
-// stream and reused by every forward and adjoint call until the nodes change.
-// Tiles with many nodes are split into chunks so one workgroup never owns more
-// than kChunk nodes.  For the plane-tiled kernels the nodes of a tile are
-// further sorted by their window start within it: row and column unit first,
-// plane last (sub_key), so nodes whose windows cover the same rows and
-// column units follow each other with rising plane; the adjoint accumulates
-// such a run in registers and writes the tile once per run.
+// Nodes sorted on the device by the grid tile that holds the start of their
+// window, u_t mod n_t, and -- for the plane-tiled kernels -- within a tile by
+// their window start there: row and column unit first, plane last (sub_key),
+// so nodes whose windows cover the same rows and column units follow each
+// other with rising plane (the adjoint accumulates such a run in registers
+// and writes the tile once per run).  The sort is a stable LSD radix sort of
+// the composite key tile * sub_range + sub_key over the node indices, so
+// nodes with equal keys keep their index order: the sorted order, and with
+// it the order of every adjoint sum, depends on the nodes only, never on
+// scheduling.  Built once per node set on the plan stream and reused by
+// every forward and adjoint call until the nodes change.  Tiles with many
+// nodes are split into chunks so one workgroup never owns more than kChunk
+// nodes.
+//
+// All device buffers live in one block: allocated by build() (own memory),
+// or handed in by bind() (a caller's region; nothing is allocated or freed).
 struct Binning {
     int built = 0;
     int usable = 0;             // tiled kernels apply to this geometry
@@ -80,31 +87,44 @@ struct Binning {
     int64_t M = 0;
     int64_t max_chunks = 0;
     int64_t slot_cells = 0;     // padded-tile cells of one adjoint slot
-    uint32_t *keys = nullptr;       // M: tile of each node
-    uint32_t *scan_part = nullptr;  // 2 x ceil(ntiles / 1024): block sums of the tile scan
+    int sort_passes = 0;        // radix passes (8-bit digits) over the composite key
+    uint32_t *keys = nullptr;       // M: composite key of each node (node order), sort ping-pong
+    uint32_t *keys2 = nullptr;      // M: sort ping-pong
     uint32_t *perm = nullptr;       // M: node at each sorted position
-    uint32_t *perm2 = nullptr;      // M: placement order before the sort within tiles
+    uint32_t *perm2 = nullptr;      // M: sort ping-pong
+    uint32_t *rs_hist = nullptr;    // 256 x radix blocks: digit counts, then their offsets
+    uint32_t *scan_part = nullptr;  // block sums of the scans
     uint32_t *tile_count = nullptr; // ntiles
     uint32_t *tile_start = nullptr; // ntiles
-    uint32_t *cursor = nullptr;     // ntiles
     uint32_t *chunk_start = nullptr;// ntiles
     uint32_t *n_chunks = nullptr;   // 1
     Chunk *chunks = nullptr;        // max_chunks
     double *xs = nullptr;           // M*d: nodes in sorted order (node-lane kernels)
     void *desc = nullptr;           // M: NodeDesc<T> in sorted order (plane-tiled kernels)
-    void *desc2 = nullptr;          // M: NodeDesc<T> in placement order (binning scratch)
+    void *desc2 = nullptr;          // M: NodeDesc<T> by node index (binning scratch)
     void *slots = nullptr;          // max_chunks padded tiles (adjoint partial sums)
     void *fs = nullptr;             // M: adjoint coefficients in sorted order
+    void *block = nullptr;          // the one allocation holding the buffers above
+    bool own_block = false;         // block came from build() (else from bind())
     int elem_bytes = 8;             // bytes of one real component (4: FP32 plan, 8: FP64)
-    bool own_scratch = true;        // allocate slots and fs (a batch brings its own)
+    bool own_scratch = true;        // include slots and fs (a batch brings its own)
     bool compensated = false;       // FP32 Kahan tile sums and Neumaier fold (CUNFFT_COMPENSATED)
 
     // Decide tile shape for this geometry; false if the tiled path does not apply.
     bool plan(const Geom &ge, int elem_bytes, int kernel_variant, int tile_variant = 0);
+    // Bytes of the device block for M nodes (plan() first).
+    size_t bytes_for(int64_t M) const;
+    // Use the caller's region [base, base + bytes) for M nodes instead of an
+    // allocation; false (nothing changed) if it is too small.  base is 256-byte
+    // aligned.  The next build() sorts into it.
+    bool bind(void *base, size_t bytes, int64_t M);
     void build(const Geom &ge, const double *x, hipStream_t s);
     template <typename T> void run_build(const Geom &ge, const BinGeom &bg, const double *x, hipStream_t s);
     void release();
     bool planes() const { return d >= 2 && variant == 0; }
+
+  private:
+    void carve(char *base, int64_t M);
 };
 
 struct BinGeom {
@@ -137,7 +157,7 @@ BinGeom bin_geom(const Binning &b);
 template <typename T> struct WinConst {
     T kb[3];    // Gaussian: FP32 log2(e) / b_t (for exp2); FP64 1 / b_t (for exp)
     T nrm[3];   // Gaussian: (pi b_t)^-1/2
-    T kbb[3];   // Kaiser-Bessel shape b_t
+    T shp[3];   // Kaiser-Bessel shape b_t, or sinc a_t
     int m;
     int win;    // Window
 };

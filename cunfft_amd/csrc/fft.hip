@@ -65,10 +65,9 @@ template <typename T> __device__ __forceinline__ void dft4(cplx_t<T> (&v)[4], in
 }
 
 struct LineInfo {
-    int64_t gbase;  // grid offset of the line start (without the column)
+    int64_t gbase;  // grid offset of the line's axis start (without the column)
     int64_t fbase;  // fhat offset of the row (passes on the last axis)
     double fac;     // product of roll-off factors of the axes before t
-    int valid;
 };
 
 // Map the line index pa (support of the axes before t, FFT order, last axis
@@ -77,8 +76,7 @@ template <typename T>
 __device__ LineInfo line_info(const FftPass<T> &P, int64_t pa)
 {
     LineInfo li;
-    li.valid = pa < P.n_pa;
-    int64_t rem = li.valid ? pa : 0;
+    int64_t rem = pa;
     int64_t a = 0, gmul = 1, fq = 0, fmul = 1;
     double fac = 1.0;
     // axes tt = t-1 .. 0, unrolled so the per-axis fields are read by constant
@@ -98,7 +96,7 @@ __device__ LineInfo line_info(const FftPass<T> &P, int64_t pa)
         fmul *= Nn;
         fac *= P.roc[P.rocoff[tt] + p];
     }
-    li.gbase = a * (int64_t)P.L * P.S;
+    li.gbase = a * P.Lf * P.So;
     li.fbase = fq * (P.t == 0 ? P.N[0] : (P.t == 1 ? P.N[1] : P.N[2]));
     li.fac = fac;
     return li;
@@ -256,16 +254,19 @@ __device__ __forceinline__ void stage_any(cplx_t<T> *buf, const cplx_t<T> *__res
     __syncthreads();
 }
 
-template <typename T, int E>
+template <typename T, int E, int SPLIT>
 __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
 {
     using C2 = cplx_t<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2 *buf = reinterpret_cast<C2 *>(smem_raw);
-    __shared__ int64_t s_gbase[kMaxRowsPerGroup];
+    __shared__ int64_t s_gbase[kMaxRowsPerGroup];   // input offset of the line (column 0)
+    __shared__ int64_t s_obase[kMaxRowsPerGroup];   // SPLIT_B: output offset of the line
     __shared__ int64_t s_fbase[kMaxRowsPerGroup];
     __shared__ double s_fac[kMaxRowsPerGroup];
     __shared__ int s_valid[kMaxRowsPerGroup];
+    __shared__ int s_k2[kMaxRowsPerGroup];          // SPLIT_B: output index k2 of the line
+    __shared__ int s_i1[kMaxRowsPerGroup];          // SPLIT_A: index i1 of the column
 
     const int tid = threadIdx.x;
     const int L = P.L, logL = P.logL, C = P.C, logC = P.logC, ldsL = P.ldsL;
@@ -280,12 +281,22 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
         s0 = (blk - pa0 * nch) << logC;
     }
     if (tid < (rows ? C : 1)) {
-        const LineInfo li = line_info<T>(P, pa0 + tid);
-        s_gbase[tid] = li.gbase;
+        const int64_t pa = pa0 + tid;
+        const bool valid = pa < P.n_pa;
+        int64_t pb = valid ? pa : 0, k2 = 0;
+        if (SPLIT == SPLIT_B) {
+            k2 = pb % P.L2;
+            pb /= P.L2;
+        }
+        const LineInfo li = line_info<T>(P, pb);
+        s_gbase[tid] = li.gbase + (SPLIT == SPLIT_B ? k2 * P.L1 * P.So : 0);
+        s_obase[tid] = li.gbase;
         s_fbase[tid] = li.fbase;
         s_fac[tid] = li.fac;
-        s_valid[tid] = li.valid;
+        s_valid[tid] = valid;
+        s_k2[tid] = (int)k2;
     }
+    if (SPLIT == SPLIT_A && tid < C) s_i1[tid] = (int)((s0 + tid) / P.So);
     // the twiddle table of the axis, staged in LDS after the lines
     C2 *twl = buf + C * ldsL;
     for (int q = tid; q < L; q += kFftThreads) twl[q] = P.tw[q];
@@ -294,10 +305,16 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
     const bool pow2 = logL >= 0;
     const float rL = 1.0f / (float)L;
     const int total = C * L;
+    const int64_t Lf = P.Lf;
     const int Nt = (int)(P.t == 0 ? P.N[0] : (P.t == 1 ? P.N[1] : P.N[2]));
     const int hN = Nt >> 1;
     const double *roct =
         P.roc + (P.t == 0 ? P.rocoff[0] : (P.t == 1 ? P.rocoff[1] : P.rocoff[2]));
+    // the full-axis index of element i of line / column c before the pass
+    auto in_index = [&](int i, int c) -> int64_t {
+        if constexpr (SPLIT == SPLIT_A) return (int64_t)s_i1[c] + P.L1 * (int64_t)i;
+        return i;
+    };
 
     // loads in batches of kFftLoadBatch per thread, all issued before the
     // first LDS store (a pass is bound by memory latency otherwise): every
@@ -330,10 +347,11 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
                 const bool in = e < total;
                 const int li = rows && in ? c : 0;
                 at[u] = in ? c * ldsL + i : -1;
-                const bool insupp = (i < hN) || (i >= L - hN);
+                const int64_t ifl = in_index(i, in ? c : 0);
+                const bool insupp = (ifl < hN) || (ifl >= Lf - hN);
                 if constexpr (FROM_FHAT) {
                     ok[u] = in && s_valid[li] && insupp;
-                    const int p = i < hN ? i : i - (L - Nt);
+                    const int p = (int)(ifl < hN ? ifl : ifl - (Lf - Nt));
                     const int q = p < hN ? p + hN : p - hN;
                     const int64_t fb = s_fbase[li];
                     const double fl = s_fac[li];
@@ -350,6 +368,17 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
                 C2 w = ok[u] ? v[u] : cmake<T>(0, 0);
                 if constexpr (FROM_FHAT)
                     w = cmake<T>((T)((double)w.x * fr[u]), (T)((double)w.y * fr[u]));
+                if constexpr (SPLIT == SPLIT_B) {
+                    // w_Lf^(sign i1 k2), the angle reduced mod Lf and taken in FP64
+                    if (at[u] >= 0) {
+                        const int c = at[u] / ldsL, i = at[u] - c * ldsL;
+                        const int64_t q = ((int64_t)i * s_k2[rows ? c : 0]) % Lf;
+                        double sn, cs;
+                        sincospi((double)P.sign * 2.0 * (double)q / (double)Lf, &sn, &cs);
+                        const double wr = (double)w.x, wi = (double)w.y;
+                        w = cmake<T>((T)(wr * cs - wi * sn), (T)(wr * sn + wi * cs));
+                    }
+                }
                 if (at[u] >= 0) buf[at[u]] = w;
             }
         }
@@ -402,45 +431,61 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
         }
         const int li = rows ? c : 0;
         if (!s_valid[li]) continue;
-        const bool insupp = (i < hN) || (i >= L - hN);
+        if constexpr (SPLIT == SPLIT_A) {
+            // the first half of a split axis: back in place, nothing pruned
+            P.g[s_gbase[li] + (int64_t)i * P.S + s0 + (rows ? 0 : c)] = buf[c * ldsL + i];
+            continue;
+        }
+        // full output index: i, or k2 + L2 k1 for the second half of a split axis
+        const int64_t kf = SPLIT == SPLIT_B ? s_k2[li] + P.L2 * (int64_t)i : (int64_t)i;
+        const bool insupp = (kf < hN) || (kf >= Lf - hN);
         if (P.mode & FFT_TO_FHAT) {
             if (insupp) {
-                const int p = i < hN ? i : i - (L - Nt);
+                const int p = (int)(kf < hN ? kf : kf - (Lf - Nt));
                 const int q = p < hN ? p + hN : p - hN;
                 const C2 v = buf[c * ldsL + i];
                 const double f = s_fac[li] * roct[p];
                 P.fhat_out[s_fbase[li] + q] = cmake<T>((T)((double)v.x * f), (T)((double)v.y * f));
             }
         } else if (!(P.mode & FFT_OUT_PRUNED) || insupp) {
-            P.g[s_gbase[li] + (int64_t)i * P.S + s0 + (rows ? 0 : c)] = buf[c * ldsL + i];
+            if constexpr (SPLIT == SPLIT_B)
+                P.dst[s_obase[li] + kf * P.So + s0 + (rows ? 0 : c)] = buf[c * ldsL + i];
+            else
+                P.g[s_gbase[li] + (int64_t)i * P.S + s0 + (rows ? 0 : c)] = buf[c * ldsL + i];
         }
     }
 }
 
-template <typename T, int E>
+template <typename T, int E, int SPLIT>
 __global__ __launch_bounds__(kFftThreads) void k_fft_pass(FftPass<T> P)
 {
-    fft_pass<T, E>(P, blockIdx.x);
+    fft_pass<T, E, SPLIT>(P, blockIdx.x);
 }
 
 // Passes of many transforms in one grid (a batch's jobs; LDS sized for the
-// largest).
+// largest).  Batches run unsplit axes only.
 template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) void k_fft_pass_batch(const FftPass<T> *__restrict__ passes,
                                                                 const BlockMap *__restrict__ map)
 {
     const BlockMap bm = map[blockIdx.x];
-    fft_pass<T, E>(passes[bm.job], bm.blk);
+    fft_pass<T, E, SPLIT_NONE>(passes[bm.job], bm.blk);
 }
 
 template <typename T>
 void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipStream_t s)
 {
     if (blocks <= 0) return;
-    if (p.C * p.L > kFftSmallElems)
-        k_fft_pass<T, fft_max_elems<T>()><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
+    if (blocks >= (int64_t(1) << 32) / kFftThreads)
+        throw Error(CUNFFT_ERR_UNSUPPORTED, "FFT pass needs too many workgroups");
+    static_assert(fft_max_elems<float>() == kFftSmallElems && fft_max_elems<double>() == kFftSmallElems,
+                  "one LDS line size");
+    if (p.split == SPLIT_A)
+        k_fft_pass<T, kFftSmallElems, SPLIT_A><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
+    else if (p.split == SPLIT_B)
+        k_fft_pass<T, kFftSmallElems, SPLIT_B><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
     else
-        k_fft_pass<T, kFftSmallElems><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
+        k_fft_pass<T, kFftSmallElems, SPLIT_NONE><<<(unsigned)blocks, kFftThreads, lds_bytes, s>>>(p);
     CUNFFT_HIP(hipGetLastError());
 }
 
@@ -479,41 +524,14 @@ static int64_t next_pow2_i(int64_t v)
 
 // Lines per workgroup: rows (S = 1) are packed up to 2048 elements per
 // workgroup; strided passes take C adjacent columns (coalesced C-wide reads)
-// up to 4096 elements.
-template <typename T>
-FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *roc,
-                            const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
-                            cplx_t<T> *fhat_out, cplx_t<T> *g)
+// up to 4096 elements.  Sets C, logC, ldsL and the launch's blocks and LDS
+// from P.L, P.S and P.n_pa.
+template <typename T> static void pass_shape(FftLaunch<T> &out)
 {
-    const int d = ge.d;
-    FftLaunch<T> out{};
     FftPass<T> &P = out.P;
-    P.g = g;
-    P.fhat_in = fhat_in;
-    P.fhat_out = fhat_out;
-    P.tw = tw;
-    P.roc = roc;
-    for (int q = 0; q < 3; ++q) {
-        P.rocoff[q] = rocoff[q];
-        P.n[q] = ge.n[q];
-        P.N[q] = ge.N[q];
-    }
-    P.d = d;
-    P.t = t;
-    P.L = (int)ge.n[t];
     P.logL = (P.L & (P.L - 1)) == 0 ? ilog2_i(P.L) : -1;
     P.n3 = 0;
     for (int l = P.L; l % 3 == 0; l /= 3) ++P.n3;
-    P.S = 1;
-    for (int q = t + 1; q < d; ++q) P.S *= ge.n[q];
-    P.n_pa = 1;
-    for (int q = 0; q < t; ++q) P.n_pa *= ge.N[q];
-    P.sign = forward ? -1 : +1;
-    if (forward)
-        P.mode = (t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED;
-    else
-        P.mode = (t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED;
-    // C (a power of two) lines per workgroup
     const int64_t target = 2048;
     int64_t C;
     if (P.S == 1) {
@@ -533,7 +551,99 @@ FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *r
     P.C = (int)C;
     P.logC = ilog2_i(C);
     out.lds = ((size_t)C * P.ldsL + P.L) * sizeof(cplx_t<T>);   // lines + twiddles
+}
+
+template <typename T>
+static FftPass<T> pass_common(const Geom &ge, const double *roc, const int64_t *rocoff, int t,
+                              bool forward, const cplx_t<T> *fhat_in, cplx_t<T> *fhat_out,
+                              cplx_t<T> *g)
+{
+    FftPass<T> P{};
+    P.g = g;
+    P.fhat_in = fhat_in;
+    P.fhat_out = fhat_out;
+    P.roc = roc;
+    for (int q = 0; q < 3; ++q) {
+        P.rocoff[q] = rocoff[q];
+        P.n[q] = ge.n[q];
+        P.N[q] = ge.N[q];
+    }
+    P.d = ge.d;
+    P.t = t;
+    P.Lf = ge.n[t];
+    P.So = 1;
+    for (int q = t + 1; q < ge.d; ++q) P.So *= ge.n[q];
+    P.n_pa = 1;
+    for (int q = 0; q < t; ++q) P.n_pa *= ge.N[q];
+    P.sign = forward ? -1 : +1;
+    P.split = SPLIT_NONE;
+    P.dst = nullptr;
+    return P;
+}
+
+template <typename T>
+FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *roc,
+                            const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
+                            cplx_t<T> *fhat_out, cplx_t<T> *g)
+{
+    const int d = ge.d;
+    FftLaunch<T> out{};
+    FftPass<T> &P = out.P;
+    P = pass_common<T>(ge, roc, rocoff, t, forward, fhat_in, fhat_out, g);
+    P.tw = tw;
+    P.L = (int)ge.n[t];
+    P.S = P.So;
+    P.L1 = 1;
+    P.L2 = P.L;
+    if (forward)
+        P.mode = (t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED;
+    else
+        P.mode = (t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED;
+    pass_shape<T>(out);
     return out;
+}
+
+template <typename T>
+int fft_axis_setup(const Geom &ge, const cplx_t<T> *const *tw, const double *roc,
+                   const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
+                   cplx_t<T> *fhat_out, cplx_t<T> **g, cplx_t<T> **g_other, FftLaunch<T> *out)
+{
+    const int d = ge.d;
+    if (fft_line_ok(ge.n[t])) {
+        out[0] = fft_pass_setup<T>(ge, tw[0], roc, rocoff, t, forward, fhat_in, fhat_out, *g);
+        return 1;
+    }
+    int64_t L1, L2;
+    if (!fft_split(ge.n[t], L1, L2)) throw Error(CUNFFT_ERR_UNSUPPORTED, "no FFT for this axis length");
+    if (!*g_other) throw Error(CUNFFT_ERR_STATE, "a split FFT axis needs the second grid buffer");
+    // A: the L2-point FFTs over i2 (stride L1 S), in place, with the input
+    // pruning (and the roll-off from fhat) of the axis
+    FftLaunch<T> &A = out[0];
+    A.P = pass_common<T>(ge, roc, rocoff, t, forward, fhat_in, fhat_out, *g);
+    A.P.split = SPLIT_A;
+    A.P.tw = tw[1];
+    A.P.L = (int)L2;
+    A.P.S = L1 * A.P.So;
+    A.P.L1 = L1;
+    A.P.L2 = L2;
+    A.P.mode = forward ? ((t == d - 1) ? FFT_FROM_FHAT : FFT_IN_PRUNED) : 0;
+    pass_shape<T>(A);
+    // B: per k2 the L1-point FFT over i1 with the w_Lf twiddles, to the other
+    // grid in natural order (or to fhat with the crop and roll-off)
+    FftLaunch<T> &B = out[1];
+    B.P = pass_common<T>(ge, roc, rocoff, t, forward, fhat_in, fhat_out, *g);
+    B.P.split = SPLIT_B;
+    B.P.tw = tw[0];
+    B.P.L = (int)L1;
+    B.P.S = B.P.So;
+    B.P.L1 = L1;
+    B.P.L2 = L2;
+    B.P.n_pa *= L2;
+    B.P.mode = forward ? 0 : ((t == d - 1) ? FFT_TO_FHAT : FFT_OUT_PRUNED);
+    B.P.dst = B.P.mode == FFT_TO_FHAT ? nullptr : *g_other;
+    pass_shape<T>(B);
+    if (B.P.mode != FFT_TO_FHAT) std::swap(*g, *g_other);
+    return 2;
 }
 
 template void launch_fft_pass<float>(const FftPass<float> &, int64_t, size_t, hipStream_t);
@@ -548,5 +658,11 @@ template FftLaunch<float> fft_pass_setup<float>(const Geom &, const float2 *, co
 template FftLaunch<double> fft_pass_setup<double>(const Geom &, const double2 *, const double *,
                                                   const int64_t *, int, bool, const double2 *,
                                                   double2 *, double2 *);
+template int fft_axis_setup<float>(const Geom &, const float2 *const *, const double *,
+                                   const int64_t *, int, bool, const float2 *, float2 *, float2 **,
+                                   float2 **, FftLaunch<float> *);
+template int fft_axis_setup<double>(const Geom &, const double2 *const *, const double *,
+                                    const int64_t *, int, bool, const double2 *, double2 *,
+                                    double2 **, double2 **, FftLaunch<double> *);
 
 }  // namespace cunfft

@@ -35,8 +35,7 @@ __device__ __forceinline__ void load_window(const Geom &ge, const double *__rest
 #pragma unroll
     for (int t = 0; t < D; ++t) {
         double c;
-        window_start(x[j * D + t], ge.n[t], ge.m, nw.u[t], c);
-        nw.u[t] = wrap_lo(nw.u[t], ge.n[t]);
+        nw.u[t] = window_cell(x[j * D + t], ge.n[t], ge.m, c);
         if constexpr (WS <= 18) {
             window_weights<T, WS>(c, ge, t, nw.w[t]);
         } else {
@@ -203,8 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_x(const cplx_t<T> *__restrict__
 #pragma unroll
     for (int t = 0; t < D; ++t) {
         double c;
-        window_start(x[j * D + t], ge.n[t], ge.m, u[t], c);
-        u[t] = wrap_lo(u[t], ge.n[t]);
+        u[t] = window_cell(x[j * D + t], ge.n[t], ge.m, c);
         if constexpr (WS <= 18) {
             window_weights_dx<T, WS>(c, ge, t, w[t], dw[t]);
         } else {

@@ -184,11 +184,11 @@ template <typename T, int D, int W, int PS, int V> struct TileLayout {
 
 // ----------------------------------------------------------- helpers ----
 
-// The Kaiser-Bessel weight out of line: the Gaussian path keeps its
+// The Kaiser-Bessel and sinc weights out of line: the Gaussian path keeps its
 // registers (the window is a run-time choice of the plan).
-template <typename T> __device__ __noinline__ T kb_phi_call(T s, T m, T b)
+template <typename T> __device__ __noinline__ T alt_phi_call(int win, T s, int m, T shape)
 {
-    return kb_phi<T>(s, m, b);
+    return win == WIN_SINC ? sinc_phi<T>(s, m, shape) : kb_phi<T>(s, (T)m, shape);
 }
 
 template <typename T> __device__ __forceinline__ T wexp(T v);
@@ -358,8 +358,8 @@ __device__ __forceinline__ NodeRec batch_prologue(WinBatch<T, D, W, NB, NWP> &wb
 #pragma unroll
         for (int ps = 0; ps < NPASS; ++ps) {
             if (lane + 64 * ps >= nb * W) break;
-            const T w = wc.win == WIN_KAISER_BESSEL
-                            ? kb_phi_call<T>(sv[ps], (T)wc.m, wc.kbb[t])
+            const T w = wc.win != WIN_GAUSSIAN
+                            ? alt_phi_call<T>(wc.win, sv[ps], wc.m, wc.shp[t])
                             : wc.nrm[t] * wexp<T>(-wc.kb[t] * sv[ps] * sv[ps]);
             if (t == 0) {
                 if constexpr (NWP > 0)
@@ -444,9 +444,9 @@ __device__ __forceinline__ NodeRec batch_prologue_wg(WinBatch<T, D, W, NB, NWP> 
             const T sv = wb.fx[n][t] + (T)(wc.m - l);
             const T kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
             const T nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
-            const T kbbt = t == 0 ? wc.kbb[0] : (t == 1 ? wc.kbb[1] : wc.kbb[2]);
-            const T w = wc.win == WIN_KAISER_BESSEL ? kb_phi_call<T>(sv, (T)wc.m, kbbt)
-                                                    : nrt * wexp<T>(-kbt * sv * sv);
+            const T shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
+            const T w = wc.win != WIN_GAUSSIAN ? alt_phi_call<T>(wc.win, sv, wc.m, shpt)
+                                               : nrt * wexp<T>(-kbt * sv * sv);
             if (t == 0) {
                 if constexpr (NWP > 0) wb.w0[n][(l % NWP) * WB::OWNP + l / NWP] = w;
                 else wb.w0[n][l] = w;
@@ -1546,7 +1546,7 @@ template <typename T> WinConst<T> win_const(const Geom &ge)
     for (int t = 0; t < 3; ++t) {
         w.kb[t] = (T)(ge.invb[t] * sc);
         w.nrm[t] = (T)ge.nrm[t];
-        w.kbb[t] = (T)ge.kbb[t];
+        w.shp[t] = (T)ge.shp[t];
     }
     w.m = ge.m;
     w.win = ge.win;

@@ -21,12 +21,36 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define CUNFFT_HIP(call) ::cunfft::hip_check((call), #call)
 
-// FFT lengths this build runs: n = 2^a 3^b with a >= 1, n <= 4096.
-inline bool fft_length_ok(int64_t n)
+// Axis lengths one LDS-resident pass transforms: L = 2^a 3^b with a >= 1,
+// L <= kFftMaxLine.  Longer axes (up to CUNFFT_MAX_AXIS) are split into two
+// such factors, L = L1 L2, and run as two passes (fft_split).
+constexpr int64_t kFftMaxLine = 4096;
+inline bool fft_line_ok(int64_t n)
 {
-    if (n < 2 || n > CUNFFT_MAX_AXIS || (n & 1)) return false;
+    if (n < 2 || n > kFftMaxLine || (n & 1)) return false;
     while (n % 3 == 0) n /= 3;
     return (n & (n - 1)) == 0;
+}
+// The factors of a split axis: L1 (the minor one, stride 1 within the axis)
+// and L2 = L / L1, both valid lines, L1 the largest such factor up to
+// sqrt(L) (balanced passes); false if there are none.
+inline bool fft_split(int64_t n, int64_t &L1, int64_t &L2)
+{
+    L1 = L2 = 0;
+    for (int64_t a = 2; a * a <= n; ++a) {
+        if (n % a || !fft_line_ok(a) || !fft_line_ok(n / a)) continue;
+        L1 = a;
+        L2 = n / a;
+    }
+    return L1 > 0;
+}
+// FFT lengths this build runs: n = 2^a 3^b with a >= 1, n <= CUNFFT_MAX_AXIS.
+inline bool fft_length_ok(int64_t n)
+{
+    if (fft_line_ok(n)) return true;
+    if (n > CUNFFT_MAX_AXIS) return false;
+    int64_t L1, L2;
+    return fft_split(n, L1, L2);
 }
 
 // Geometry of one transform (window shape from sigma_t = n_t / N_t).
@@ -39,12 +63,14 @@ double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3]);
 
 // ----------------------------------------------------------------- FFT ---
 // Target of a cunfftHandle: twiddle tables tw[q] = exp(-2 pi i q / n_t) per
-// axis, computed in extended precision on the host.
+// axis, computed in extended precision on the host -- or, for an axis longer
+// than kFftMaxLine, the tables of its two factors L1 (tw) and L2 (tw2).
 struct FftTables {
     int prec;
     int d;
     int64_t n[3];
     void *tw[3];
+    void *tw2[3];
 };
 FftTables *make_fft_tables(int prec, int d, const int64_t *n);
 void free_fft_tables(void *h);
@@ -61,6 +87,17 @@ enum FftMode : int {
     FFT_TO_FHAT = 8,     // write fhat (centred order) times the roll-off factor
 };
 
+// A split axis of length Lf = L1 L2 (element i = i1 + L1 i2, output
+// k = k2 + L2 k1) runs as two passes:
+//   SPLIT_A  the L2-point FFTs over i2 for every i1 (stride L1 S), in place:
+//            Y[i1 + L1 k2] = sum_i2 x[i1 + L1 i2] w_L2^(i2 k2);
+//   SPLIT_B  per k2: the L1-point FFT over i1 of Y[k2 L1 + i1] w_Lf^(i1 k2),
+//            written to X[k2 + L2 k1] of another grid (dst) in natural
+//            order, or to fhat with the crop and roll-off.
+// The pruning of the first forward / last adjoint pass and the roll-off use
+// the full axis index.
+enum FftSplit : int { SPLIT_NONE = 0, SPLIT_A = 1, SPLIT_B = 2 };
+
 template <typename T> struct FftPass {
     cplx_t<T> *g;
     const cplx_t<T> *fhat_in;
@@ -74,11 +111,17 @@ template <typename T> struct FftPass {
     int L, logL;             // logL = log2(L) for a power of two, else -1
     int n3;                  // radix-3 factors of L (L = 2^a 3^n3)
     int64_t S;
-    int64_t n_pa;            // number of lines over the axes before t
+    int64_t n_pa;            // number of lines over the axes before t (times L2 for SPLIT_B)
     int C, logC;             // lines per workgroup
     int ldsL;                // LDS row pitch in elements
     int sign;                // -1 forward, +1 adjoint
     int mode;
+    // split axes
+    int split;               // FftSplit
+    int64_t Lf;              // the axis length (L when not split)
+    int64_t L1, L2;          // its factors (split passes)
+    int64_t So;              // the axis stride (product of the later axes)
+    cplx_t<T> *dst;          // SPLIT_B output grid (not for FFT_TO_FHAT)
 };
 
 template <typename T>
@@ -86,7 +129,8 @@ void launch_fft_pass(const FftPass<T> &p, int64_t blocks, size_t lds_bytes, hipS
 
 // Pass along axis t of one transform (forward: axes d-1 .. 0, the first from
 // fhat with the roll-off; adjoint: axes 0 .. d-1, the last to fhat), with its
-// launch shape.  tw is the twiddle table of length n_t.
+// launch shape.  tw is the twiddle table of length n_t.  Axes longer than
+// kFftMaxLine need fft_axis_setup.
 template <typename T> struct FftLaunch {
     FftPass<T> P;
     int64_t blocks;
@@ -96,6 +140,14 @@ template <typename T>
 FftLaunch<T> fft_pass_setup(const Geom &ge, const cplx_t<T> *tw, const double *roc,
                             const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
                             cplx_t<T> *fhat_out, cplx_t<T> *g);
+// The passes of axis t (one, or two for a split axis): tw[0] is the twiddle
+// table of the axis (n_t), or of L1 and tw[1] of L2 when it is split.  The
+// axis's data is in *g; a split axis leaves it in g_other and swaps the two
+// pointers (unless it writes fhat).  Returns the number of passes (<= 2).
+template <typename T>
+int fft_axis_setup(const Geom &ge, const cplx_t<T> *const *tw, const double *roc,
+                   const int64_t *rocoff, int t, bool forward, const cplx_t<T> *fhat_in,
+                   cplx_t<T> *fhat_out, cplx_t<T> **g, cplx_t<T> **g_other, FftLaunch<T> *out);
 
 // Workgroup b of a batched launch works on block map[b].blk of job map[b].job.
 struct BlockMap {

@@ -8,6 +8,8 @@
 // a fixed-order tree reduction combines them, so results are reproducible and
 // a few thousand sampled outputs still fill the chip.  Terms use sincospi on
 // the FP64 phase; FP32 plans read FP32 inputs and round only the result.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace cunfft {
@@ -37,16 +39,28 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *sa, dou
     b = sb[0];
 }
 
+// An output index outside [0, M) (rows) or [0, N_total) (columns) gives NaN
+// for that output; the check is uniform over the workgroup, before any barrier.
+template <typename T> __device__ __forceinline__ cplx_t<T> nan_c()
+{
+    return cmake<T>((T)__builtin_nan(""), (T)__builtin_nan(""));
+}
+
 template <typename T>
-__global__ __launch_bounds__(kNdftThreads) void k_ndft_rows(NdftDims nd, uint32_t Nt,
+__global__ __launch_bounds__(kNdftThreads) void k_ndft_rows(NdftDims nd, uint32_t Nt, int64_t M,
                                                             const double *__restrict__ x,
                                                             const cplx_t<T> *__restrict__ fhat,
                                                             const long long *__restrict__ rows,
+                                                            int64_t r0,
                                                             cplx_t<T> *__restrict__ out)
 {
     __shared__ double sa[kNdftThreads], sb[kNdftThreads];
-    const int64_t r = blockIdx.x;
+    const int64_t r = r0 + blockIdx.x;
     const int64_t j = rows ? rows[r] : r;
+    if (j < 0 || j >= M) {
+        if (threadIdx.x == 0) out[r] = nan_c<T>();
+        return;
+    }
     double xj[3] = {0, 0, 0};
     for (int t = 0; t < nd.d; ++t) xj[t] = x[j * nd.d + t];
     double sr = 0, si = 0;
@@ -71,11 +85,17 @@ __global__ __launch_bounds__(kNdftThreads) void k_ndft_cols(NdftDims nd, int64_t
                                                             const double *__restrict__ x,
                                                             const cplx_t<T> *__restrict__ f,
                                                             const long long *__restrict__ cols,
+                                                            int64_t r0, int64_t Nt,
                                                             cplx_t<T> *__restrict__ out)
 {
     __shared__ double sa[kNdftThreads], sb[kNdftThreads];
-    const int64_t r = blockIdx.x;
-    const uint32_t kl = (uint32_t)(cols ? cols[r] : r);
+    const int64_t r = r0 + blockIdx.x;
+    const int64_t kc = cols ? cols[r] : r;
+    if (kc < 0 || kc >= Nt) {
+        if (threadIdx.x == 0) out[r] = nan_c<T>();
+        return;
+    }
+    const uint32_t kl = (uint32_t)kc;
     double k[3] = {0, 0, 0};
     for (int t = 0; t < nd.d; ++t)
         k[t] = (double)((int)((kl / nd.sN[t]) % nd.N[t]) - (int)(nd.N[t] >> 1));
@@ -110,17 +130,23 @@ static NdftDims make_dims(int d, const int64_t *N)
     return nd;
 }
 
+// One workgroup per output, launched in pieces of kNdftMaxBlocks workgroups so
+// the thread count of a launch stays below 2^32.
+constexpr int64_t kNdftMaxBlocks = int64_t(1) << 22;
+
 template <typename T>
 void launch_ndft_rows(int d, const int64_t *N, int64_t M, const double *x, const cplx_t<T> *fhat,
                       const long long *rows, int64_t n_rows, cplx_t<T> *out, hipStream_t s)
 {
-    (void)M;
     if (n_rows <= 0) return;
     NdftDims nd = make_dims(d, N);
     uint32_t Nt = 1;
     for (int t = 0; t < d; ++t) Nt *= (uint32_t)N[t];
-    k_ndft_rows<T><<<(unsigned)n_rows, kNdftThreads, 0, s>>>(nd, Nt, x, fhat, rows, out);
-    CUNFFT_HIP(hipGetLastError());
+    for (int64_t r0 = 0; r0 < n_rows; r0 += kNdftMaxBlocks) {
+        const int64_t nb = std::min<int64_t>(kNdftMaxBlocks, n_rows - r0);
+        k_ndft_rows<T><<<(unsigned)nb, kNdftThreads, 0, s>>>(nd, Nt, M, x, fhat, rows, r0, out);
+        CUNFFT_HIP(hipGetLastError());
+    }
 }
 
 template <typename T>
@@ -129,8 +155,13 @@ void launch_ndft_cols(int d, const int64_t *N, int64_t M, const double *x, const
 {
     if (n_cols <= 0) return;
     NdftDims nd = make_dims(d, N);
-    k_ndft_cols<T><<<(unsigned)n_cols, kNdftThreads, 0, s>>>(nd, M, x, f, cols, out);
-    CUNFFT_HIP(hipGetLastError());
+    int64_t Nt = 1;
+    for (int t = 0; t < d; ++t) Nt *= N[t];
+    for (int64_t r0 = 0; r0 < n_cols; r0 += kNdftMaxBlocks) {
+        const int64_t nb = std::min<int64_t>(kNdftMaxBlocks, n_cols - r0);
+        k_ndft_cols<T><<<(unsigned)nb, kNdftThreads, 0, s>>>(nd, M, x, f, cols, r0, Nt, out);
+        CUNFFT_HIP(hipGetLastError());
+    }
 }
 
 template void launch_ndft_rows<float>(int, const int64_t *, int64_t, const double *,

@@ -55,10 +55,20 @@ FftTables *make_fft_tables(int prec, int d, const int64_t *n)
     for (int t = 0; t < 3; ++t) {
         ft->n[t] = t < d ? n[t] : 1;
         ft->tw[t] = nullptr;
+        ft->tw2[t] = nullptr;
     }
     try {
-        for (int t = 0; t < d; ++t) {
-            const int64_t L = n[t];
+        // one table per line length: the axis, or its two factors when it is split
+        for (int k = 0; k < 2 * d; ++k) {
+            const int t = k / 2;
+            int64_t L = n[t], L1 = 0, L2 = 0;
+            if (!fft_line_ok(L)) {
+                if (!fft_split(L, L1, L2)) throw Error(CUNFFT_ERR_UNSUPPORTED, "FFT length");
+                L = k % 2 == 0 ? L1 : L2;
+            } else if (k % 2 == 1) {
+                continue;
+            }
+            void *&dst = k % 2 == 0 ? ft->tw[t] : ft->tw2[t];
             const size_t es = prec == 64 ? sizeof(double2) : sizeof(float2);
             std::vector<unsigned char> host((size_t)L * es);
             for (int64_t q = 0; q < L; ++q) {
@@ -78,12 +88,14 @@ FftTables *make_fft_tables(int prec, int d, const int64_t *n)
                     memcpy(host.data() + q * es, &v, es);
                 }
             }
-            CUNFFT_HIP(hipMalloc(&ft->tw[t], host.size()));
-            CUNFFT_HIP(hipMemcpy(ft->tw[t], host.data(), host.size(), hipMemcpyHostToDevice));
+            CUNFFT_HIP(hipMalloc(&dst, host.size()));
+            CUNFFT_HIP(hipMemcpy(dst, host.data(), host.size(), hipMemcpyHostToDevice));
         }
     } catch (...) {
-        for (int t = 0; t < 3; ++t)
+        for (int t = 0; t < 3; ++t) {
             if (ft->tw[t]) (void)hipFree(ft->tw[t]);
+            if (ft->tw2[t]) (void)hipFree(ft->tw2[t]);
+        }
         delete ft;
         throw;
     }
@@ -94,8 +106,10 @@ void free_fft_tables(void *h)
 {
     FftTables *ft = static_cast<FftTables *>(h);
     if (!ft) return;
-    for (int t = 0; t < 3; ++t)
+    for (int t = 0; t < 3; ++t) {
         if (ft->tw[t]) (void)hipFree(ft->tw[t]);
+        if (ft->tw2[t]) (void)hipFree(ft->tw2[t]);
+    }
     delete ft;
 }
 
@@ -112,6 +126,11 @@ template <typename T> struct Impl {
     bool dirty[2] = {true, true};
     int config[2] = {0, 0};  // gridding kernel configuration per direction
     bool compensated = false;  // CUNFFT_COMPENSATED on an FP32 plan
+    // a second grid for the axes longer than one FFT pass (their second pass
+    // writes the other buffer), and the buffer the last transform left its
+    // grid in (copy_g_ToHost)
+    cplx_t<T> *g2 = nullptr;
+    cplx_t<T> *g_last = nullptr;
     int slot(int dir) const { return config[1] == config[0] ? 0 : dir; }
     void nodes_changed() { dirty[0] = dirty[1] = true; }
 };
@@ -148,13 +167,17 @@ template <typename T> static void validate(const typename Traits<T>::plan *p)
         if (p->N[t] < 2 || (p->N[t] & 1))
             throw Error(CUNFFT_ERR_INVALID_ARG, "N_t must be even and >= 2");
         if (!fft_length_ok(p->n[t]))
-            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t must be 2^a 3^b with a >= 1 (and <= 4096)");
+            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t must be 2^a 3^b with a >= 1 (and <= 2^24)");
         if (p->n[t] < p->N[t]) throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= N_t");
         if (p->n[t] < 2 * p->m + 2)
             throw Error(CUNFFT_ERR_INVALID_ARG, "n_t must be >= 2m+2 (window wider than grid)");
-        if (p->n[t] > CUNFFT_MAX_AXIS)
-            throw Error(CUNFFT_ERR_UNSUPPORTED, "n_t above 4096 is not supported");
+        // the sinc window's roll-off 1 / M_2m(k / (n a)) is infinite at the band
+        // edge when sigma_t = 1
+        if ((p->flags & CUNFFT_SINC) && p->n[t] <= p->N[t])
+            throw Error(CUNFFT_ERR_INVALID_ARG, "CUNFFT_SINC needs n_t > N_t");
     }
+    if ((p->flags & CUNFFT_SINC) && (p->flags & CUNFFT_KAISER_BESSEL))
+        throw Error(CUNFFT_ERR_INVALID_ARG, "CUNFFT_SINC and CUNFFT_KAISER_BESSEL exclude each other");
 }
 
 template <typename T> static void zero_fields(typename Traits<T>::plan *p)
@@ -162,8 +185,15 @@ template <typename T> static void zero_fields(typename Traits<T>::plan *p)
     memset((void *)p, 0, sizeof(*p));
 }
 
+template <typename T> struct Impl;
+
 template <typename T> static void free_device(typename Traits<T>::plan *p)
 {
+    if (p->impl) {
+        Impl<T> *im = static_cast<Impl<T> *>(p->impl);
+        if (im->g2) (void)hipFree(im->g2);
+        im->g2 = im->g_last = nullptr;
+    }
     if (p->g_gpu) (void)hipFree(p->g_gpu);
     if (p->fhat_gpu) (void)hipFree(p->fhat_gpu);
     if (p->x_gpu) (void)hipFree(p->x_gpu);
@@ -187,7 +217,14 @@ template <typename T> static void init_device_para(typename Traits<T>::plan *p)
     CUNFFT_HIP(hipMemset(p->fhat_gpu, 0, sizeof(C2) * p->N_total));
     CUNFFT_HIP(hipMemset(p->x_gpu, 0, sizeof(double) * p->M_total * p->d));
     CUNFFT_HIP(hipMemset(p->f_gpu, 0, sizeof(C2) * p->M_total));
-    if (p->impl) static_cast<Impl<T> *>(p->impl)->nodes_changed();
+    if (p->impl) {
+        Impl<T> *im = static_cast<Impl<T> *>(p->impl);
+        im->nodes_changed();
+        bool split = false;
+        for (int t = 0; t < p->d; ++t) split = split || !fft_line_ok(p->n[t]);
+        if (split) CUNFFT_HIP(hipMalloc((void **)&im->g2, sizeof(C2) * p->n_total));
+        im->g_last = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
+    }
 }
 
 // Window shape b_t = 2 sigma_t / (2 sigma_t - 1) * m / pi with sigma_t = n_t / N_t
@@ -208,10 +245,12 @@ Geom make_geom(int d, const int64_t *N, const int64_t *n, int64_t M, int m, int 
             const double b = (2.0 * sigma) / (2.0 * sigma - 1.0) * ((double)m / M_PI);
             ge.invb[t] = 1.0 / b;
             ge.nrm[t] = 1.0 / sqrt(M_PI * b);
-            // Kaiser-Bessel shape of NFFT3: b = pi (2 - 1 / sigma)
-            ge.kbb[t] = M_PI * (2.0 - 1.0 / sigma);
+            // Kaiser-Bessel shape of NFFT3: b = pi (2 - 1 / sigma); sinc power:
+            // a = (2 sigma - 1) / (2 sigma m)
+            ge.shp[t] = win == WIN_SINC ? (2.0 * sigma - 1.0) / (2.0 * sigma * (double)m)
+                                        : M_PI * (2.0 - 1.0 / sigma);
         } else {
-            ge.invb[t] = ge.nrm[t] = ge.kbb[t] = 0.0;
+            ge.invb[t] = ge.nrm[t] = ge.shp[t] = 0.0;
         }
     }
     return ge;
@@ -230,8 +269,28 @@ static double bessel_i0(double z)
     return (double)sum;
 }
 
+// Centred cardinal B-spline M_p(x) of order p (support [-p/2, p/2], unit
+// integral), the Fourier transform of the sinc-power window: de Boor's
+// recursion on the knots 0..p at t = x + p/2, over the p splines of order 1
+// around the knot interval of t only.
+static double bspline_centred(int p, double x)
+{
+    const long double t = (long double)x + 0.5L * p;
+    if (t <= 0.0L || t >= (long double)p) return 0.0;
+    const int j = (int)floorl(t);
+    long double B[2 * CUNFFT_MAX_CUT_OFF + 2] = {};   // B[i] = B_k^(i)(t), i <= j
+    B[j] = 1.0L;
+    for (int k = 2; k <= p; ++k)
+        for (int i = j - k + 1 < 0 ? 0 : j - k + 1; i <= j; ++i) {
+            const long double right = i + 1 <= p - 1 ? B[i + 1] : 0.0L;
+            B[i] = ((t - i) * B[i] + (i + k - t) * right) / (k - 1);
+        }
+    return (double)B[0];
+}
+
 // Device roll-off tables by pruned index q < N_t: exp(fac_t k^2) with
-// k = q or q - N_t and fac_t = pi^2 b_t / n_t^2 (src/cunfft.cu:79-90).
+// k = q or q - N_t and fac_t = pi^2 b_t / n_t^2 (src/cunfft.cu:79-90), or
+// 1 / phihat_t(k / n_t) of the other windows.
 double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3])
 {
     std::vector<double> roc;
@@ -246,8 +305,11 @@ double *make_rolloff(const Geom &ge, int64_t (&rocoff)[3])
             const int64_t k = q < Nn / 2 ? q : q - Nn;
             if (ge.win == WIN_KAISER_BESSEL) {
                 const double w = 2.0 * M_PI * (double)k / (double)ge.n[t];
-                const double z = (double)ge.m * sqrt(ge.kbb[t] * ge.kbb[t] - w * w);
+                const double z = (double)ge.m * sqrt(ge.shp[t] * ge.shp[t] - w * w);
                 roc.push_back(1.0 / bessel_i0(z));
+            } else if (ge.win == WIN_SINC) {
+                const double nu = (double)k / ((double)ge.n[t] * ge.shp[t]);
+                roc.push_back(1.0 / bspline_centred(2 * ge.m, nu));
             } else {
                 roc.push_back(exp(fac * (double)k * (double)k));
             }
@@ -272,10 +334,17 @@ static void fg_exp_l(double *out, double b, int offset, int m)
 // with tile shape c, c == kTileVariants the node-lane kernels.
 static bool config_exists(int d, int c) { return c >= 0 && c <= kTileVariants && (d >= 2 || c == 0); }
 
+// False when the tiled kernels of configuration c do not apply; a
+// compensated FP32 plan of d >= 2 also refuses the node-lane configuration,
+// which has no compensated variant.
 static bool plan_bins(Binning &b, const Geom &ge, int elem_bytes, int c, bool comp)
 {
     const bool lanes = c == kTileVariants;
     b.compensated = comp && elem_bytes == 4;
+    if (b.compensated && lanes && ge.d >= 2) {
+        b.usable = 0;
+        return false;
+    }
     return b.plan(ge, elem_bytes, lanes ? 1 : 0, lanes ? 0 : c);
 }
 
@@ -348,10 +417,23 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
         nn3[t] = p->n[t];
     }
     im->geom = make_geom(d, NN, nn3, p->M_total, p->m,
-                         (p->flags & CUNFFT_KAISER_BESSEL) ? WIN_KAISER_BESSEL : WIN_GAUSSIAN);
+                         (p->flags & CUNFFT_KAISER_BESSEL) ? WIN_KAISER_BESSEL
+                         : (p->flags & CUNFFT_SINC)        ? WIN_SINC
+                                                           : WIN_GAUSSIAN);
     const int c0 = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
     im->compensated = (p->flags & CUNFFT_COMPENSATED) && sizeof(T) == 4;
-    plan_configs<T>(im, c0, c0);
+    bool tiled = plan_configs<T>(im, c0, c0);
+    if (im->compensated) {
+        // the compensated sums live in the tiled adjoint kernels only (the
+        // untiled fallback adds with global atomics): take the first tile
+        // shape that holds the doubled tile, or refuse the plan
+        for (int c = 1; c < kTileVariants && !tiled && c0 == 0 && d >= 2; ++c)
+            tiled = plan_configs<T>(im, c, c);
+        if (!tiled)
+            throw Error(CUNFFT_ERR_UNSUPPORTED,
+                        "CUNFFT_COMPENSATED: no compensated tiled gridding configuration fits this "
+                        "plan's window and grid (or CUNFFT_NODE_LANES was asked for)");
+    }
     im->roc = make_rolloff(im->geom, im->rocoff);
     init_device_para<T>(p);
     int64_t nn[3];
@@ -398,18 +480,27 @@ template <typename T> static void finalize(typename Traits<T>::plan *p)
 }
 
 // ------------------------------------------------------------ FFT passes --
+// The d axes' passes on grid g (and the plan's second grid for split axes);
+// returns the buffer holding the grid after them.
 template <typename T>
-static void fft_passes(typename Traits<T>::plan *p, Impl<T> *im, FftTables *ft, bool forward,
-                       const cplx_t<T> *fhat_in, cplx_t<T> *fhat_out, cplx_t<T> *g, hipStream_t s)
+static cplx_t<T> *fft_passes(typename Traits<T>::plan *p, Impl<T> *im, FftTables *ft, bool forward,
+                             const cplx_t<T> *fhat_in, cplx_t<T> *fhat_out, cplx_t<T> *g,
+                             hipStream_t s)
 {
     const int d = p->d;
+    cplx_t<T> *cur = g;
+    cplx_t<T> *other = im->g2 == g ? reinterpret_cast<cplx_t<T> *>(p->g_gpu) : im->g2;
     for (int k = 0; k < d; ++k) {
         const int t = forward ? d - 1 - k : k;
-        const FftLaunch<T> L = fft_pass_setup<T>(im->geom, static_cast<const cplx_t<T> *>(ft->tw[t]),
-                                                 im->roc, im->rocoff, t, forward, fhat_in,
-                                                 fhat_out, g);
-        launch_fft_pass<T>(L.P, L.blocks, L.lds, s);
+        const cplx_t<T> *tw[2] = {static_cast<const cplx_t<T> *>(ft->tw[t]),
+                                  static_cast<const cplx_t<T> *>(ft->tw2[t])};
+        FftLaunch<T> L[2];
+        const int np = fft_axis_setup<T>(im->geom, tw, im->roc, im->rocoff, t, forward, fhat_in,
+                                         fhat_out, &cur, &other, L);
+        for (int i = 0; i < np; ++i) launch_fft_pass<T>(L[i].P, L[i].blocks, L[i].lds, s);
     }
+    im->g_last = cur;
+    return cur;
 }
 
 template <typename T> static FftTables *fft_tables(typename Traits<T>::plan *p, bool forward)
@@ -441,10 +532,10 @@ template <typename T>
 static void run_forward(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T> *fhat,
                         cplx_t<T> *f, hipStream_t s, bool timing, double *t_fft, double *t_conv)
 {
-    cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
     const double *x = p->x_gpu;
     if (timing) CUNFFT_HIP(hipEventRecord(im->ev[0], s));
-    fft_passes<T>(p, im, fft_tables<T>(p, true), true, fhat, nullptr, g, s);
+    const cplx_t<T> *g = fft_passes<T>(p, im, fft_tables<T>(p, true), true, fhat, nullptr,
+                                       reinterpret_cast<cplx_t<T> *>(p->g_gpu), s);
     if (timing) CUNFFT_HIP(hipEventRecord(im->ev[1], s));
     if (p->flags & CUNFFT_PLAIN_GATHER) {
         launch_gather_plain<T>(im->geom, g, x, f, s);
@@ -761,6 +852,17 @@ void api_copy_dev_to(typename Traits<T>::plan *p, void *dst, const void *src, si
     });
 }
 
+// The grid of the last transform (the buffer its FFT passes left it in: the
+// plan's grid, or its second grid after an odd number of split axes).
+template <typename T> void api_copy_g(typename Traits<T>::plan *p)
+{
+    if (!p) return;
+    const void *src = p->impl && static_cast<Impl<T> *>(p->impl)->g_last
+                          ? static_cast<const void *>(static_cast<Impl<T> *>(p->impl)->g_last)
+                          : static_cast<const void *>(p->g_gpu);
+    api_copy_dev_to<T>(p, p->g, src, sizeof(typename Traits<T>::hc) * p->n_total);
+}
+
 template <typename T> void api_transform(typename Traits<T>::plan *p)
 {
     guard(p, [&] {
@@ -892,9 +994,9 @@ int api_grad_x_dev(typename Traits<T>::plan *p, const void *fhat, const void *v,
         need_device<T>(p);
         if (!fhat || !v || !gx) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
         hipStream_t s = pick_stream(stream, im->stream);
-        cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
-        fft_passes<T>(p, im, fft_tables<T>(p, true), true, static_cast<const cplx_t<T> *>(fhat),
-                      nullptr, g, s);
+        const cplx_t<T> *g = fft_passes<T>(p, im, fft_tables<T>(p, true), true,
+                                           static_cast<const cplx_t<T> *>(fhat), nullptr,
+                                           reinterpret_cast<cplx_t<T> *>(p->g_gpu), s);
         launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(v), gx, s);
     });
 }
@@ -915,9 +1017,9 @@ int api_backward_dev(typename Traits<T>::plan *p, const void *fhat, const void *
         if (gx && !fhat) throw Error(CUNFFT_ERR_INVALID_ARG, "the x-gradient needs fhat");
         hipStream_t s = pick_stream(stream, im->stream);
         if (gx) {
-            cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
-            fft_passes<T>(p, im, fft_tables<T>(p, true), true,
-                          static_cast<const cplx_t<T> *>(fhat), nullptr, g, s);
+            const cplx_t<T> *g = fft_passes<T>(p, im, fft_tables<T>(p, true), true,
+                                               static_cast<const cplx_t<T> *>(fhat), nullptr,
+                                               reinterpret_cast<cplx_t<T> *>(p->g_gpu), s);
             launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(gf), gx, s);
         }
         if (gfhat) {
@@ -936,6 +1038,7 @@ int api_ndft_dev(typename Traits<T>::plan *p, bool adjoint, const void *in, cons
         Impl<T> *im = impl_of<T>(p);
         need_device<T>(p);
         if (!in || !out) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        if (idx && n_idx < 0) throw Error(CUNFFT_ERR_INVALID_ARG, "negative output count");
         if (p->N_total > 0x7fffffff) throw Error(CUNFFT_ERR_UNSUPPORTED, "NDFT needs N_total < 2^31");
         int64_t N[3];
         for (int t = 0; t < p->d; ++t) N[t] = p->N[t];
@@ -988,7 +1091,9 @@ template <typename T> int api_set_config(typename Traits<T>::plan *p, int dirs, 
     }
     Binning probe;
     if (!plan_bins(probe, im->geom, (int)sizeof(T), c, im->compensated)) {
-        report("the tiled kernels of this configuration do not fit the plan's window and grid");
+        report(im->compensated && c == kTileVariants
+                   ? "CUNFFT_COMPENSATED plans have no node-lane configuration"
+                   : "the tiled kernels of this configuration do not fit the plan's window and grid");
         return CUNFFT_ERR_UNSUPPORTED;
     }
     return guard(p, [&] {
@@ -1073,10 +1178,7 @@ void copy_f_ToHost(cunfft_plan *p, cunfftDoubleComplex *arr)
 {
     if (p) api_copy_dev_to<double>(p, arr, p->f_gpu, sizeof(*arr) * p->M_total);
 }
-void copy_g_ToHost(cunfft_plan *p)
-{
-    if (p) api_copy_dev_to<double>(p, p->g, p->g_gpu, sizeof(cunfftDoubleComplex) * p->n_total);
-}
+void copy_g_ToHost(cunfft_plan *p) { api_copy_g<double>(p); }
 void copy_x_toHost(cunfft_plan *p, double *arr)
 {
     if (p) api_copy_dev_to<double>(p, arr, p->x_gpu, sizeof(double) * p->M_total * p->d);
@@ -1236,10 +1338,7 @@ void cunfftf_copy_f_ToHost(cunfftf_plan *p, cunfftFloatComplex *arr)
 {
     if (p) api_copy_dev_to<float>(p, arr, p->f_gpu, sizeof(*arr) * p->M_total);
 }
-void cunfftf_copy_g_ToHost(cunfftf_plan *p)
-{
-    if (p) api_copy_dev_to<float>(p, p->g, p->g_gpu, sizeof(cunfftFloatComplex) * p->n_total);
-}
+void cunfftf_copy_g_ToHost(cunfftf_plan *p) { api_copy_g<float>(p); }
 void cunfftf_copy_x_toHost(cunfftf_plan *p, double *arr)
 {
     if (p) api_copy_dev_to<float>(p, arr, p->x_gpu, sizeof(double) * p->M_total * p->d);

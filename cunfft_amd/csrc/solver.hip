@@ -132,9 +132,11 @@ void solve_cgnr(const SolverOps<T> &ops, const cplx_t<T> *y, const T *w, cplx_t<
     using C = cplx_t<T>;
     if (max_iter < 0) throw Error(CUNFFT_ERR_INVALID_ARG, "max_iter must be >= 0");
     C *r = nullptr, *v = nullptr, *z = nullptr, *p = nullptr;
+    C *fbest = nullptr;   // the iterate of the smallest checked |z|
     double *part = nullptr, *scal = nullptr;
     auto release = [&] {
-        for (void *q : {(void *)r, (void *)v, (void *)z, (void *)p, (void *)part, (void *)scal})
+        for (void *q : {(void *)r, (void *)v, (void *)z, (void *)p, (void *)fbest, (void *)part,
+                        (void *)scal})
             if (q) (void)hipFree(q);
     };
     try {
@@ -142,6 +144,7 @@ void solve_cgnr(const SolverOps<T> &ops, const cplx_t<T> *y, const T *w, cplx_t<
         CUNFFT_HIP(hipMalloc((void **)&v, sizeof(C) * M));
         CUNFFT_HIP(hipMalloc((void **)&z, sizeof(C) * N));
         CUNFFT_HIP(hipMalloc((void **)&p, sizeof(C) * N));
+        CUNFFT_HIP(hipMalloc((void **)&fbest, sizeof(C) * N));
         CUNFFT_HIP(hipMalloc((void **)&part, sizeof(double) * kRedBlocks));
         const int SL_ZH = SL_RR + max_iter + 1;   // |z|^2 history
         CUNFFT_HIP(hipMalloc((void **)&scal, sizeof(double) * (SL_ZH + max_iter + 1)));
@@ -160,6 +163,7 @@ void solve_cgnr(const SolverOps<T> &ops, const cplx_t<T> *y, const T *w, cplx_t<
         CUNFFT_HIP(hipMemcpyAsync(&rr, scal + SL_RR, sizeof(double), hipMemcpyDeviceToHost, s));
         CUNFFT_HIP(hipMemcpyAsync(&zz0, scal + SL_ZH, sizeof(double), hipMemcpyDeviceToHost, s));
         CUNFFT_HIP(hipStreamSynchronize(s));
+        CUNFFT_HIP(hipMemcpyAsync(fbest, f, sizeof(C) * N, hipMemcpyDeviceToDevice, s));
         const double ynorm = yy > 0 ? sqrt(yy) : 1.0;
         const double z0 = sqrt(zz0);
         double zmin = z0;
@@ -187,8 +191,16 @@ void solve_cgnr(const SolverOps<T> &ops, const cplx_t<T> *y, const T *w, cplx_t<
                                           hipMemcpyDeviceToHost, s));
                 CUNFFT_HIP(hipStreamSynchronize(s));
                 zk = sqrt(zk);
-                stop = sqrt(rr) <= tol * ynorm || zk <= tol * z0 || zk > 10.0 * zmin;
-                zmin = zk < zmin ? zk : zmin;
+                const bool diverged = zk > 10.0 * zmin;
+                stop = sqrt(rr) <= tol * ynorm || zk <= tol * z0 || diverged;
+                if (diverged) {
+                    // loss of orthogonality: return the best checked iterate, not
+                    // the diverging one
+                    CUNFFT_HIP(hipMemcpyAsync(f, fbest, sizeof(C) * N, hipMemcpyDeviceToDevice, s));
+                } else if (zk < zmin) {
+                    zmin = zk;
+                    CUNFFT_HIP(hipMemcpyAsync(fbest, f, sizeof(C) * N, hipMemcpyDeviceToDevice, s));
+                }
             }
         }
         std::vector<double> hist((size_t)done + 1);

@@ -41,8 +41,10 @@ extern "C" {
 #define CUNFFT_DEFAULT_CUT_OFF 6
 #define CUNFFT_MAX_CUT_OFF 16
 #define CUNFFT_MAX_DIM 3
-/** Largest oversampled axis length n_t handled by one LDS-resident pass. */
-#define CUNFFT_MAX_AXIS 4096
+/** Largest oversampled axis length n_t (= 2^a 3^b, a >= 1).  Axes up to
+ *  4096 run as one LDS-resident pass each; longer ones as two (n_t = L1 L2,
+ *  both factors up to 4096). */
+#define CUNFFT_MAX_AXIS 16777216
 
 /* plan flags, same bits as src/cunfft_typedefs.h:46-53 */
 #define CUFFT_INIT  (1U << 0)
@@ -68,7 +70,10 @@ extern "C" {
  *  the plane-tiled accumulation (a second LDS tile holds the compensation
  *  terms) and Neumaier sums in the fold.  Costs LDS space and time; FP64
  *  plans ignore it.  Configurations whose doubled tile does not fit the
- *  LDS (FP32 3D 16x16x8) are not offered. */
+ *  LDS (FP32 3D 16x16x8, FP32 3D m >= 6) and the node-lane configuration
+ *  are not offered; a plan with none left (FP32 3D m >= 6, or
+ *  CUNFFT_NODE_LANES as well) is refused with CUNFFT_ERR_UNSUPPORTED.  1D
+ *  plans accumulate their adjoint tiles in FP64 anyway. */
 #define CUNFFT_COMPENSATED    (1U << 13)
 /** Asynchronous copy boundary: copyDataToDevice[Ad], cunfft_transform and
  *  cunfft_adjoint return once their work is queued on the plan's stream
@@ -83,6 +88,12 @@ extern "C" {
  *  the same 2m+2 taps per axis reach ~1e-7 at m = 4 and ~1e-14 at m = 8
  *  against the NDFT, where the Gaussian reaches ~1e-4 and ~1e-8. */
 #define CUNFFT_KAISER_BESSEL  (1U << 15)
+/** Sinc-power window instead of the reference's Gaussian (NFFT3's
+ *  SINC_POWER): phi(s) = a sinc(pi a s)^(2m) in grid units with
+ *  a = (2 sigma - 1) / (2 sigma m), Fourier transform the centred cardinal
+ *  B-spline of order 2m, roll-off 1 / M_2m(k / (n a)).  Needs n_t > N_t;
+ *  excludes CUNFFT_KAISER_BESSEL. */
+#define CUNFFT_SINC           (1U << 16)
 
 /* status codes */
 #define CUNFFT_SUCCESS          0
@@ -275,7 +286,9 @@ int cunfft_synchronize(cunfft_plan *plan);
  * |y - A fhat|_W <= tol |y|_W, or once the normal-equation residual
  * |A^H W (y - A fhat)| falls to tol times its start (the least-squares
  * solution of an inconsistent system) or grows tenfold over its smallest
- * value (loss of orthogonality).  res_hist (NULL or max_iter + 1 doubles)
+ * value (loss of orthogonality; fhat is then the iterate of the smallest
+ * |A^H W (y - A fhat)| seen, which is checked every 8 iterations).
+ * res_hist (NULL or max_iter + 1 doubles)
  * receives the relative residual of each iteration, *iters (may be NULL)
  * the iterations done.  Allocates its work vectors and frees them; runs on
  * `stream` and synchronizes it. */
@@ -321,7 +334,9 @@ int cunfft_get_config(const cunfft_plan *plan, int direction);
  * it was. */
 int cunfft_set_config(cunfft_plan *plan, int directions, int config);
 /* Direct NDFT on the GPU in FP64 for the listed outputs only (rows of f or
- * entries of fhat by flat index); rows == NULL means all outputs. */
+ * entries of fhat by flat index); rows == NULL means all outputs.  An index
+ * outside [0, M_total) (rows) or [0, N_total) (columns) yields NaN for that
+ * output; a negative count is CUNFFT_ERR_INVALID_ARG. */
 int cundft_transform_rows_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
                               const long long *rows_dev, long long n_rows,
                               cunfftDoubleComplex *out_dev, void *stream);
@@ -363,8 +378,10 @@ int cunfft_batch_buffers(cunfft_batch *batch, int problem, void **fhat_in, void 
  * synchronizes `stream` once. */
 int cunfft_batch_execute(cunfft_batch *batch, int directions, void *stream);
 /* Autotuner over the batch as a whole (cunfft_autotune for batches): times
- * each gridding configuration's graph per direction and keeps the fastest;
- * ms_out as for cunfft_autotune.  Synchronizes `stream`. */
+ * each gridding configuration's graph per direction on nonzero inputs and
+ * keeps the fastest; ms_out as for cunfft_autotune.  The input buffers are
+ * saved before and restored after (temporary device copies); the output
+ * buffers are overwritten.  Synchronizes `stream`. */
 int cunfft_batch_autotune(cunfft_batch *batch, int directions, int reps, float *ms_out,
                           void *stream);
 int cunfft_batch_get_config(const cunfft_batch *batch, int direction);

@@ -56,6 +56,13 @@ def lib():
             fn.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.orc_bessel_i0.restype = ctypes.c_double
         L.orc_bessel_i0.argtypes = [ctypes.c_double]
+        L.orc_sinc_shape.restype = ctypes.c_double
+        L.orc_sinc_shape.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+        for fn in (L.orc_phi_sinc, L.orc_dphi_sinc):
+            fn.restype = ctypes.c_double
+            fn.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_bspline.restype = ctypes.c_double
+        L.orc_bspline.argtypes = [ctypes.c_int, ctypes.c_double]
         L.orc_rel_linf.restype = ctypes.c_double
         L.orc_rel_l2.restype = ctypes.c_double
         _lib = L
@@ -115,12 +122,12 @@ class Drand48:
         return f, x
 
 
-WINDOWS = {"gaussian": 0, "kaiser_bessel": 1}
+WINDOWS = {"gaussian": 0, "kaiser_bessel": 1, "sinc": 2}
 
 
 def set_window(kind: str) -> None:
-    """Window of the oracle's transforms: "gaussian" (the reference's) or
-    "kaiser_bessel" (NFFT3's).  Process-wide; tests restore "gaussian"."""
+    """Window of the oracle's transforms: "gaussian" (the reference's),
+    "kaiser_bessel" or "sinc" (NFFT3's).  Process-wide; tests restore "gaussian"."""
     lib().orc_set_window(WINDOWS[kind])
 
 
@@ -152,6 +159,24 @@ def dphi_kb(s, m, b):
 
 def bessel_i0(z):
     return lib().orc_bessel_i0(float(z))
+
+
+def sinc_shape(N, n, m):
+    """a = (2 sigma - 1) / (2 sigma m) of the sinc-power window."""
+    return lib().orc_sinc_shape(int(N), int(n), int(m))
+
+
+def phi_sinc(s, m, a):
+    return lib().orc_phi_sinc(float(s), int(m), float(a))
+
+
+def dphi_sinc(s, m, a):
+    return lib().orc_dphi_sinc(float(s), int(m), float(a))
+
+
+def bspline(p, x):
+    """Centred cardinal B-spline M_p(x) of order p."""
+    return lib().orc_bspline(int(p), float(x))
 
 
 def shape_b(N, n, m):

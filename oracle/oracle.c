@@ -89,7 +89,7 @@ double orc_phi(double t, double b)
  * Both branches are one power series in d = m^2 - s^2,
  *   phi = (1/pi) sum_j b^{2j+1} d^j / (2j+1)!,
  * used near d = 0. */
-static int g_window = 0;   /* 0 Gaussian, 1 Kaiser-Bessel */
+static int g_window = 0;   /* 0 Gaussian, 1 Kaiser-Bessel, 2 sinc power */
 
 void orc_set_window(int kind) { g_window = kind; }
 int orc_get_window(void) { return g_window; }
@@ -121,6 +121,62 @@ double orc_dphi_kb(double s, int m, double b)
     return s * (br * cos(br) - sin(br)) / (M_PI * r * r * r);
 }
 
+/* ---------------------------------------------------------- sinc power -- */
+/* NFFT3's SINC_POWER window (the reference is Gaussian-only) in grid units
+ * s = n x - l:  phi(s) = a sinc(pi a s)^(2m), sinc(u) = sin(u)/u, with
+ * a = (2 sigma - 1) / (2 sigma m).  Its Fourier transform (frequency nu in
+ * cycles per cell) is the centred cardinal B-spline M_2m(nu / a) of order 2m,
+ * which vanishes for |nu| >= m a = 1 - 1/(2 sigma), the first alias of the
+ * band edge of the oversampled grid. */
+double orc_sinc_shape(int64_t N, int64_t n, int m)
+{
+    const double sigma = (double)n / (double)N;
+    return (2.0 * sigma - 1.0) / (2.0 * sigma * (double)m);
+}
+
+static double sinc_of(double u) { return u == 0.0 ? 1.0 : sin(u) / u; }
+
+double orc_phi_sinc(double s, int m, double a)
+{
+    return a * pow(sinc_of(M_PI * a * s), 2 * m);
+}
+
+/* d sinc / du = (u cos u - sin u) / u^2, by its Taylor series near 0. */
+static double dsinc_of(double u)
+{
+    if (fabs(u) < 0.1) {
+        const double u2 = u * u;
+        return u * (-1.0 / 3.0 + u2 * (1.0 / 30.0 + u2 * (-1.0 / 840.0 + u2 / 45360.0)));
+    }
+    return (u * cos(u) - sin(u)) / (u * u);
+}
+
+double orc_dphi_sinc(double s, int m, double a)
+{
+    const double u = M_PI * a * s;
+    return a * 2.0 * m * pow(sinc_of(u), 2 * m - 1) * dsinc_of(u) * M_PI * a;
+}
+
+/* Centred cardinal B-spline of order p: the p-fold convolution of the unit
+ * box, support [-p/2, p/2], by its explicit sum
+ *   M_p(x) = 1/(p-1)! sum_{j=0..p} (-1)^j C(p, j) (x + p/2 - j)_+^(p-1)
+ * in long double, at -|x| (M_p is even).  The alternating sum cancels:
+ * long double keeps ~1e-13 up to p = 16 (m = 8), less beyond. */
+double orc_bspline(int p, double x)
+{
+    /* M_p is even: the left half has the fewest (and smallest) terms */
+    const long double t = 0.5L * p - fabsl((long double)x);
+    if (t <= 0.0L || t >= (long double)p) return 0.0;
+    long double sum = 0.0L, binom = 1.0L, fact = 1.0L;
+    for (int k = 2; k < p; ++k) fact *= k;
+    for (int j = 0; j <= p; ++j) {
+        const long double v = t - j;
+        if (v > 0.0L) sum += ((j & 1) ? -binom : binom) * powl(v, p - 1);
+        binom = binom * (p - j) / (j + 1);
+    }
+    return (double)(sum / fact);
+}
+
 /* Modified Bessel I0 by its power series in long double. */
 double orc_bessel_i0(double z)
 {
@@ -139,6 +195,10 @@ static double rolloff_factor(int64_t N, int64_t n, int m, int64_t k)
     if (g_window == 1) {
         const double b = orc_kb_shape(N, n), w = 2.0 * M_PI * (double)k / (double)n;
         return 1.0 / orc_bessel_i0((double)m * sqrt(b * b - w * w));
+    }
+    if (g_window == 2) {
+        const double a = orc_sinc_shape(N, n, m);
+        return 1.0 / orc_bspline(2 * m, (double)k / ((double)n * a));
     }
     const double b = orc_shape_b(N, n, m);
     return exp(M_PI * M_PI * b / ((double)n * (double)n) * (double)k * (double)k);
@@ -317,6 +377,7 @@ static void node_window(int d, const int64_t *N, const int64_t *n, int m,
     for (int t = 0; t < d; ++t) {
         double b = orc_shape_b(N[t], n[t], m);
         double bk = orc_kb_shape(N[t], n[t]);
+        double as = orc_sinc_shape(N[t], n[t], m);
         int64_t u = orc_window_start(n[t], xj[t], m);
         double nx = (double)n[t] * xj[t];
         for (int l = 0; l < W; ++l) {
@@ -325,6 +386,9 @@ static void node_window(int d, const int64_t *N, const int64_t *n, int m,
             if (g_window == 1) {
                 nw->w[t][l] = orc_phi_kb(s, m, bk);
                 if (want_dw) nw->dw[t][l] = (double)n[t] * orc_dphi_kb(s, m, bk);
+            } else if (g_window == 2) {
+                nw->w[t][l] = orc_phi_sinc(s, m, as);
+                if (want_dw) nw->dw[t][l] = (double)n[t] * orc_dphi_sinc(s, m, as);
             } else {
                 double p = orc_phi(s, b);
                 nw->w[t][l] = p;

@@ -52,8 +52,9 @@ int64_t orc_window_start(int64_t n, double x, int m);
 double orc_phi(double t, double b);
 
 /* Window of the transforms below: 0 Gaussian (the reference's, default),
- * 1 Kaiser-Bessel (NFFT3's, which the reference links for its CPU side;
- * no counterpart in the reference's GPU code).  Process-wide setting. */
+ * 1 Kaiser-Bessel, 2 sinc power (NFFT3's, which the reference links for its
+ * CPU side; no counterpart in the reference's GPU code).  Process-wide
+ * setting. */
 void orc_set_window(int kind);
 int orc_get_window(void);
 /* Kaiser-Bessel shape b = pi (2 - 1/sigma), window phi(s) in grid units,
@@ -62,6 +63,14 @@ double orc_kb_shape(int64_t N, int64_t n);
 double orc_phi_kb(double s, int m, double b);
 double orc_dphi_kb(double s, int m, double b);
 double orc_bessel_i0(double z);
+/* Sinc-power window (window kind 2, NFFT3's SINC_POWER): shape
+ * a = (2 sigma - 1) / (2 sigma m), phi(s) = a sinc(pi a s)^(2m), d phi / d s,
+ * and the centred cardinal B-spline M_p of order p (its Fourier transform is
+ * M_2m(nu / a)). */
+double orc_sinc_shape(int64_t N, int64_t n, int m);
+double orc_phi_sinc(double s, int m, double a);
+double orc_dphi_sinc(double s, int m, double a);
+double orc_bspline(int p, double x);
 
 /* Forward roll-off + zero pad: fhat (N_total) -> ghat (n_total), zeros
  * outside the centred box (src/cunfft_wrapper.cu:32-75,

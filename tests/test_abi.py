@@ -8,14 +8,17 @@ import os
 import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "cunfft.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("cunfft.h", "cunfft_util.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)          # comments
-    text = re.sub(r"^\s*#.*$", "", text, flags=re.M)            # preprocessor lines
-    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char)\s*\**\s*(\w+)\s*\(", text, flags=re.M)
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)          # comments
+        text = re.sub(r"^\s*#.*$", "", text, flags=re.M)            # preprocessor lines
+        names += re.findall(r"^\s*(?:const\s+)?(?:void|int|char|double|uint_t|GPUTimer)\s*\**\s*(\w+)\s*\(",
+                            text, flags=re.M)
     return sorted(set(names))
 
 

@@ -74,3 +74,44 @@ def test_fp64_plans_ignore_the_flag(gpu, orc):
     fv = r.standard_normal(M) + 1j * r.standard_normal(M)
     a = adjoint(gpu, N, n, m, x, fv, "double", gpu.COMPENSATED)
     assert orc.rel_l2(orc.nfft_adjoint(N, n, m, x, fv), a) <= 1e-12
+
+
+def test_compensated_plan_without_a_fitting_configuration_is_refused(gpu):
+    """FP32 3D at m = 6: no tile shape holds the doubled (Kahan) tile, so the
+    plan is refused instead of running uncompensated atomics."""
+    with pytest.raises(gpu.CunfftError) as e:
+        gpu.Plan([16, 16, 16], 1000, m=6, precision="single",
+                 flags=gpu.CUFFT_INIT | gpu.CUFFT_BOTH | gpu.COMPENSATED)
+    assert e.value.status == 2
+    # ... and with the node-lane kernels asked for (no compensated variant)
+    with pytest.raises(gpu.CunfftError) as e:
+        gpu.Plan([32, 32], 1000, m=4, precision="single",
+                 flags=gpu.CUFFT_INIT | gpu.CUFFT_BOTH | gpu.COMPENSATED | gpu.NODE_LANES)
+    assert e.value.status == 2
+
+
+def test_compensated_plan_refuses_the_node_lane_configuration(gpu):
+    with gpu.Plan([32, 32], 1000, m=4, precision="single",
+                  flags=gpu.CUFFT_INIT | gpu.CUFFT_BOTH | gpu.COMPENSATED) as p:
+        with pytest.raises(gpu.CunfftError) as e:
+            p.config = gpu.TILE_SHAPES
+        assert e.value.status == 2
+        assert p.config == (0, 0)
+        r = np.random.default_rng(2)
+        p.x[:] = r.uniform(-0.5, 0.5, (1000, 2))
+        p.copyDataToDevice()
+        cfg, fwd, adj = p.autotune(reps=1)
+        assert adj[gpu.TILE_SHAPES] < 0 and fwd[gpu.TILE_SHAPES] < 0
+        assert gpu.TILE_SHAPES not in cfg
+
+
+def test_compensated_3d_m5_matches_the_oracle(gpu, orc):
+    N, n, m, M = [16, 16, 16], [32, 32, 32], 5, 20000
+    r = np.random.default_rng(17)
+    x = r.uniform(-0.5, 0.5, (M, 3))
+    fv = r.standard_normal(M) + 1j * r.standard_normal(M)
+    ref = orc.nfft_adjoint(N, n, m, x, fv.astype(np.complex64).astype(np.complex128))
+    h = adjoint(gpu, N, n, m, x, fv, "single", gpu.COMPENSATED)
+    err = orc.rel_l2(ref, h)
+    print("3D m=5 compensated rel-l2 vs oracle", err)
+    assert err <= 1e-5

@@ -167,3 +167,31 @@ def test_cfg3_sampled_ndft(gpu, cfg3):
     wh = np.linalg.norm(h64[cols] - nh.cpu().numpy()) / np.linalg.norm(nh.cpu().numpy())
     print("cfg3 window error on 4096 samples", wf, wh)
     assert wf <= 3.5e-4 and wh <= 3.5e-4
+
+
+def test_cfg3_gpu_ndft_matches_the_cpu_oracle(gpu, orc, cfg3):
+    """The GPU NDFT that the sampled window-error check above relies on,
+    tied to the CPU oracle at the benchmark's own size: 256 rows and 256
+    columns of the 2^21 x 128^3 NDFT (5.4e8 terms each way)."""
+    import torch
+
+    x, fh, fv = cfg3
+    N = [128] * 3
+    r = np.random.default_rng(71)
+    rows = np.sort(r.choice(x.shape[0], 256, replace=False)).astype(np.int64)
+    cols = np.sort(r.choice(128 ** 3, 256, replace=False)).astype(np.int64)
+    dev = torch.device("cuda")
+    with gpu.Plan(N, x.shape[0], n=[256] * 3, m=4, precision="double") as p:
+        p.x[:] = x
+        p.copyDataToDevice()
+        fh_t, fv_t = torch.from_numpy(fh).to(dev), torch.from_numpy(fv).to(dev)
+        rows_t, cols_t = torch.from_numpy(rows).to(dev), torch.from_numpy(cols).to(dev)
+        nf = torch.empty(256, dtype=torch.complex128, device=dev)
+        nh = torch.empty(256, dtype=torch.complex128, device=dev)
+        p.ndft_rows_dev(fh_t.data_ptr(), rows_t.data_ptr(), 256, nf.data_ptr())
+        p.ndft_cols_dev(fv_t.data_ptr(), cols_t.data_ptr(), 256, nh.data_ptr())
+        torch.cuda.synchronize()
+    ef = orc.rel_l2(orc.ndft_trafo(N, x, fh, rows=rows), nf.cpu().numpy())
+    eh = orc.rel_l2(orc.ndft_adjoint(N, x, fv, cols=cols), nh.cpu().numpy())
+    print("cfg3 GPU NDFT vs CPU oracle: rows", ef, "columns", eh)
+    assert ef <= 1e-12 and eh <= 1e-12

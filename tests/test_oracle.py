@@ -272,13 +272,16 @@ def test_planted_mistakes_fail_the_band(orc, mistake):
 
 
 # ------------------------------------------- reference's recorded bands ----
-def _bands():
+def _bands(lo=0, hi=14):
+    """Recorded rows with lo <= log2 N_total <= hi (one per direction, m, d, size);
+    the larger ones run on the GPU (tests/test_gpu_reference_bands.py)."""
     with open(os.path.join(HERE, "golden", "reference_err_bands.json")) as fh:
         rows = json.load(fh)["rows"]
     seen, out = set(), []
     for r in rows:
         key = (r["direction"], r["m"], r["d"], r["log2_N_total"])
-        if key in seen or r["log2_N_total"] > 12 or r["log2_N_total"] % r["d"]:
+        L = r["log2_N_total"]
+        if key in seen or L > hi or L < lo or L % r["d"]:
             continue
         seen.add(key)
         out.append(r)
@@ -420,6 +423,86 @@ def test_kaiser_bessel_gradient_matches_differences(orc):
     x = r.uniform(-0.4, 0.4, (20, 2))
     fh = crandn(r, 256)
     with orc.window("kaiser_bessel"):
+        dfdx = orc.nfft_dx(N, n, m, x, fh)
+        h = 1e-7
+        for j, t in [(0, 0), (3, 1), (7, 0), (11, 1)]:
+            xp, xm = x.copy(), x.copy()
+            xp[j, t] += h
+            xm[j, t] -= h
+            fd = (orc.nfft_trafo(N, n, m, xp, fh)[j] - orc.nfft_trafo(N, n, m, xm, fh)[j]) / (2 * h)
+            assert abs(fd - dfdx[j, t]) <= 1e-6 * abs(dfdx[j, t]) + 1e-8
+
+
+# ------------------------------------------------------------ sinc power --
+def test_bspline_matches_scipy(orc):
+    """The sinc window's Fourier transform, the centred cardinal B-spline,
+    against scipy's B-spline basis element on the same unit knots."""
+    from scipy.interpolate import BSpline
+
+    for p in [2, 3, 4, 6, 8, 12, 16]:
+        b = BSpline.basis_element(np.arange(p + 1) - p / 2, extrapolate=False)
+        for x in np.linspace(-p / 2 + 1e-3, p / 2 - 1e-3, 41):
+            assert abs(orc.bspline(p, x) - float(b(x))) <= 1e-14
+        assert orc.bspline(p, p / 2 + 0.1) == 0.0 and orc.bspline(p, -p / 2) == 0.0
+
+
+@pytest.mark.parametrize("m", [2, 3, 4])
+def test_sinc_window_fourier_pair(orc, m):
+    """phi(s) = a sinc(pi a s)^(2m) has Fourier transform M_2m(nu / a): checked
+    by quadrature of the oracle's window (a wrong shape a, power or
+    normalisation breaks it)."""
+    a = orc.sinc_shape(64, 128, m)
+    assert abs(a - 3.0 / (4.0 * m)) <= 1e-16
+    S, h = 400.0 / a, 0.01 / a
+    s = np.arange(-S, S + h / 2, h)
+    phi = np.array([orc.phi_sinc(v, m, a) for v in s])
+    for nu in [0.0, 0.05, 0.1, 0.2, 0.3, 0.37]:
+        ft = np.sum(phi * np.cos(2 * np.pi * nu * s)) * h
+        assert abs(ft - orc.bspline(2 * m, nu / a)) <= 1e-9
+
+
+def test_sinc_window_derivative(orc):
+    m, a = 4, orc.sinc_shape(64, 128, 4)
+    for s in [0.0, 1e-3, 0.05, 0.4, 1.3, 2.9, 4.6, -3.2]:
+        hh = 1e-6
+        fd = (orc.phi_sinc(s + hh, m, a) - orc.phi_sinc(s - hh, m, a)) / (2 * hh)
+        assert abs(fd - orc.dphi_sinc(s, m, a)) <= 1e-8
+        assert orc.phi_sinc(s, m, a) == orc.phi_sinc(-s, m, a)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_sinc_nfft_converges_to_the_ndft(orc, d):
+    """The sinc-power NFFT converges to the NDFT geometrically in m, and at
+    m = 4 is an order of magnitude closer than the Gaussian."""
+    N = {1: [64], 2: [32, 32], 3: [16, 16, 16]}[d]
+    n = [2 * v for v in N]
+    r = rng(50 + d)
+    x = r.uniform(-0.5, 0.5, (300, d))
+    fh = crandn(r, int(np.prod(N)))
+    fv = crandn(r, 300)
+    nd, na = orc.ndft_trafo(N, x, fh), orc.ndft_adjoint(N, x, fv)
+    bands = {2: 6e-3, 3: 4e-4, 4: 2.5e-5, 6: 1e-7, 8: 4e-10}
+    with orc.window("sinc"):
+        errs = []
+        for m, band in bands.items():
+            ef = orc.rel_l2(nd, orc.nfft_trafo(N, n, m, x, fh))
+            ea = orc.rel_l2(na, orc.nfft_adjoint(N, n, m, x, fv))
+            assert ef <= band and ea <= band, (m, ef, ea)
+            errs.append(ef)
+        assert all(b < a / 5 for a, b in zip(errs, errs[1:]))
+        lhs = np.vdot(fv, orc.nfft_trafo(N, n, 4, x, fh))
+        rhs = np.vdot(orc.nfft_adjoint(N, n, 4, x, fv), fh)
+        assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+        e_sinc = orc.rel_l2(nd, orc.nfft_trafo(N, n, 4, x, fh))
+    assert orc.rel_l2(nd, orc.nfft_trafo(N, n, 4, x, fh)) > 5 * e_sinc
+
+
+def test_sinc_gradient_matches_differences(orc):
+    N, n, m = [16, 16], [32, 32], 4
+    r = rng(60)
+    x = r.uniform(-0.4, 0.4, (20, 2))
+    fh = crandn(r, 256)
+    with orc.window("sinc"):
         dfdx = orc.nfft_dx(N, n, m, x, fh)
         h = 1e-7
         for j, t in [(0, 0), (3, 1), (7, 0), (11, 1)]:

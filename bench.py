@@ -315,9 +315,9 @@ def load_batch(b, problem, dev):
         xt = torch.from_numpy(x).to(dev)
         xs.append(xt)
         b.set_nodes_dev(i, xt.data_ptr())
-        fhat_in, _, f_in, _ = b.tensors(i)
-        fhat_in.copy_(torch.from_numpy(fh.reshape(-1)).to(dev))
-        f_in.copy_(torch.from_numpy(fv.reshape(-1)).to(dev))
+        fhat_in, _, f_in, _ = b.tensors(i)   # (R, N_total) and (R, M) views
+        fhat_in.copy_(torch.from_numpy(fh).to(dev))
+        f_in.copy_(torch.from_numpy(fv).to(dev))
     torch.cuda.synchronize()
     return xs
 

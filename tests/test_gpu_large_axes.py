@@ -86,11 +86,11 @@ def test_split_axis_untiled_kernels_and_gradient(gpu, orc):
     assert orc.rel_l2(orc.nfft_adjoint(N, n, m, x, fv), h) <= 1e-12
     dev = torch.device("cuda")
     with gpu.Plan(N, M, m=m) as p:
-        xt = torch.from_numpy(x).to(dev)
+        # the device arrays stay referenced until the kernels have run
+        xt, fht, fvt = torch.from_numpy(x).to(dev), torch.from_numpy(fh).to(dev), torch.from_numpy(fv).to(dev)
         gx = torch.zeros((M, 1), dtype=torch.float64, device=dev)
         p.set_nodes_dev(xt.data_ptr())
-        p.grad_x_dev(torch.from_numpy(fh).to(dev).data_ptr(), torch.from_numpy(fv).to(dev).data_ptr(),
-                     gx.data_ptr())
+        p.grad_x_dev(fht.data_ptr(), fvt.data_ptr(), gx.data_ptr())
         torch.cuda.synchronize()
     ref = (np.conj(fv)[:, None] * orc.nfft_dx(N, n, m, x, fh)).real
     assert np.linalg.norm(gx.cpu().numpy() - ref) / np.linalg.norm(ref) <= 1e-12

@@ -238,7 +238,8 @@ def test_measured_times(gpu):
 def test_errors_do_not_crash(gpu):
     cu = gpu
     for kw in [dict(N=[30], M=10, n=[40]), dict(N=[32], M=0), dict(N=[32], M=10, m=0),
-               dict(N=[8], M=10, n=[8], m=4), dict(N=[32], M=10, n=[8192])]:
+               dict(N=[8], M=10, n=[8], m=4), dict(N=[32], M=10, n=[2 ** 25]),
+               dict(N=[32], M=10, n=[8200])]:
         with pytest.raises(cu.CunfftError):
             cu.Plan(kw.pop("N"), kw.pop("M"), **kw)
     p = cu.Plan([16], 10)

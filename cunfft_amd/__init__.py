@@ -143,6 +143,8 @@ def lib():
               "cunfftf_adjoint_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, vp]
         getattr(L, s).restype = ctypes.c_int
+    L.cunfft_adjoint_noalloc_dev.argtypes = [PP, vp, vp, vp]
+    L.cunfft_adjoint_noalloc_dev.restype = ctypes.c_int
     for s in ("cunfft_grad_x_dev", "cunfftf_grad_x_dev"):
         getattr(L, s).argtypes = [PP, vp, vp, vp, vp]
         getattr(L, s).restype = ctypes.c_int
@@ -393,6 +395,14 @@ class Plan:
 
     def adjoint_dev(self, f_ptr, fhat_ptr, stream=None):
         self._dev("cunfft_adjoint_dev", "cunfftf_adjoint_dev", ctypes.c_void_p(f_ptr),
+                  ctypes.c_void_p(fhat_ptr), ctypes.c_void_p(stream))
+
+    def adjoint_noalloc_dev(self, f_ptr, fhat_ptr, stream=None):
+        """The adjoint with no device allocation (cunfft_adjoint_noalloc_dev;
+        FP64 plans): its scratch is the library's per-GPU array."""
+        if self._sp:
+            raise CunfftError(1, "cunfft_adjoint_noalloc_dev is an FP64 entry point")
+        self._dev("cunfft_adjoint_noalloc_dev", "cunfft_adjoint_noalloc_dev", ctypes.c_void_p(f_ptr),
                   ctypes.c_void_p(fhat_ptr), ctypes.c_void_p(stream))
 
     def grad_x_dev(self, fhat_ptr, v_ptr, gx_ptr, stream=None):

@@ -172,6 +172,13 @@ void launch_grad_x(const Geom &ge, const cplx_t<T> *g, const double *x, const cp
                    T *gx, hipStream_t s);
 template <typename T> void launch_zero(cplx_t<T> *p, int64_t n, hipStream_t s);
 
+// ------------------------------------------------------------ scratch ---
+// Bytes of the device-resident scratch of the allocation-free adjoint.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// Device address of the current device's scratch (hipGetSymbolAddress on the
+// first call per device, kept after).
+void *noalloc_scratch();
+
 // -------------------------------------------------------------- solver ---
 // The transforms the inverse NFFT iterates (device pointers, on stream s).
 template <typename T> struct SolverOps {

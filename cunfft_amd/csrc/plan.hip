@@ -439,6 +439,15 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
     int64_t nn[3];
     for (int t = 0; t < d; ++t) nn[t] = p->n[t];
     const int prec = sizeof(T) == 8 ? 64 : 32;
+    if (sizeof(T) == 8) {
+        // set-up of the allocation-free adjoint's scratch on this device (its
+        // address is taken once per device; a failure is reported by that
+        // entry point, not here)
+        try {
+            (void)noalloc_scratch();
+        } catch (const Error &) {
+        }
+    }
     if (p->flags & CUFFT_INIT) {
         if (p->flags & (CUFFT_BOTH | CUFFT_FORW)) p->fft_forward = make_fft_tables(prec, d, nn);
         if (p->flags & (CUFFT_BOTH | CUFFT_BACKW)) p->fft_backward = make_fft_tables(prec, d, nn);
@@ -986,6 +995,48 @@ int api_adjoint_dev(typename Traits<T>::plan *p, const void *f, void *fhat, void
     });
 }
 
+// The adjoint of api_adjoint_dev with its node binning, sorted coefficients
+// and partial-tile slots in the device-resident scratch (scratch.hip) instead
+// of the plan's lazily allocated binning: no allocation.  The binning is
+// rebuilt on every call (the scratch keeps whatever the last call on this
+// device left); every buffer the kernels read is written first in the call.
+template <typename T>
+int api_adjoint_noalloc_dev(typename Traits<T>::plan *p, const void *f, void *fhat, void *stream)
+{
+    return guard(p, [&] {
+        Impl<T> *im = impl_of<T>(p);
+        need_device<T>(p);
+        if (!fhat || !f) throw Error(CUNFFT_ERR_INVALID_ARG, "NULL device pointer");
+        FftTables *ft = static_cast<FftTables *>(p->fft_backward);
+        if (!ft)
+            throw Error(CUNFFT_ERR_STATE, "the allocation-free adjoint needs the plan's adjoint FFT "
+                                          "tables (CUFFT_INIT with CUFFT_BOTH or CUFFT_BACKW)");
+        hipStream_t s = pick_stream(stream, im->stream);
+        cplx_t<T> *g = reinterpret_cast<cplx_t<T> *>(p->g_gpu);
+        const cplx_t<T> *fin = static_cast<const cplx_t<T> *>(f);
+        Binning b;   // describes the scratch; owns nothing
+        if (!(p->flags & CUNFFT_ATOMIC_SCATTER) &&
+            plan_bins(b, im->geom, (int)sizeof(T), im->config[1], im->compensated)) {
+            const size_t need = b.bytes_for(p->M_total);
+            if (need > kNoAllocScratchBytes) {
+                char msg[200];
+                snprintf(msg, sizeof msg,
+                         "the allocation-free adjoint needs %.1f MiB of scratch for this plan; it "
+                         "holds %.0f MiB",
+                         need / 1048576.0, kNoAllocScratchBytes / 1048576.0);
+                throw Error(CUNFFT_ERR_UNSUPPORTED, msg);
+            }
+            if (!b.bind(noalloc_scratch(), kNoAllocScratchBytes, p->M_total))
+                throw Error(CUNFFT_ERR_STATE, "scratch binding");
+            b.build(im->geom, p->x_gpu, s);
+        }
+        // an unusable binning makes launch_scatter_binned clear the grid and add
+        // with atomics: no scratch either
+        launch_scatter_binned<T>(im->geom, b, p->x_gpu, fin, g, s, nullptr);
+        fft_passes<T>(p, im, ft, false, nullptr, static_cast<cplx_t<T> *>(fhat), g, s);
+    });
+}
+
 template <typename T>
 int api_grad_x_dev(typename Traits<T>::plan *p, const void *fhat, const void *v, T *gx, void *stream)
 {
@@ -1282,6 +1333,11 @@ int cunfft_adjoint_dev(cunfft_plan *p, const cunfftDoubleComplex *f, cunfftDoubl
                        void *s)
 {
     return api_adjoint_dev<double>(p, f, fhat, s);
+}
+int cunfft_adjoint_noalloc_dev(cunfft_plan *p, const cunfftDoubleComplex *f,
+                               cunfftDoubleComplex *fhat, void *s)
+{
+    return api_adjoint_noalloc_dev<double>(p, f, fhat, s);
 }
 int cunfft_grad_x_dev(cunfft_plan *p, const cunfftDoubleComplex *fhat, const cunfftDoubleComplex *v,
                       double *gx, void *s)

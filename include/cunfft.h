@@ -261,6 +261,19 @@ int cunfft_transform_dev(cunfft_plan *plan, const cunfftDoubleComplex *fhat_dev,
 /* fhat = A^H f (adjoint NFFT, src/cunfft.cu:209-224). */
 int cunfft_adjoint_dev(cunfft_plan *plan, const cunfftDoubleComplex *f_dev,
                        cunfftDoubleComplex *fhat_dev, void *stream);
+/* fhat = A^H f like cunfft_adjoint_dev, without any device allocation (no
+ * counterpart in the reference; for callers that cannot allocate: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The node binning, sorted coefficients and partial tiles live in a
+ * device-resident scratch array of 64 MiB that the library holds per GPU
+ * until the process ends; a plan that needs more (roughly M_total d 8 plus
+ * the tiles' slots, e.g. 2D 256^2 with 65536 nodes takes ~21 MiB) is refused
+ * with CUNFFT_ERR_UNSUPPORTED.  The nodes are re-binned on every call.
+ * Calls that overlap on one GPU (other streams, other threads, other plans)
+ * share the array and race: serialize them.  Needs the plan's adjoint FFT
+ * tables (CUFFT_INIT with CUFFT_BOTH or CUFFT_BACKW), else CUNFFT_ERR_STATE. */
+int cunfft_adjoint_noalloc_dev(cunfft_plan *plan, const cunfftDoubleComplex *f_dev,
+                               cunfftDoubleComplex *fhat_dev, void *stream);
 /* Backward of f = A(x) fhat with respect to x: gx[j*d+t] =
  * Re(conj(v_j) * d f_j / d x_jt) for upstream v, the derivative-of-Gaussian
  * gather over the forward grid of fhat (weights recomputed, not stored). */

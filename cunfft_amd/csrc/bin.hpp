@@ -158,6 +158,8 @@ template <typename T> struct WinConst {
     T kb[3];    // Gaussian: FP32 log2(e) / b_t (for exp2); FP64 1 / b_t (for exp)
     T nrm[3];   // Gaussian: (pi b_t)^-1/2
     T shp[3];   // Kaiser-Bessel shape b_t, or sinc a_t
+    T dk[3];    // Gaussian: -2 n_t / b_t (the weight's x-derivative is dk s phi(s))
+    T nn[3];    // n_t
     int m;
     int win;    // Window
 };
@@ -226,5 +228,10 @@ void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cp
 template <typename T>
 void launch_tile_spread(const Geom &ge, const Binning &b, const cplx_t<T> *f, cplx_t<T> *g,
                         hipStream_t s, hipEvent_t mid);
+// x-gradient gx[j d + t] = Re(conj(v_j) d f_j / d x_jt) over the forward grid g
+// on a plane-tiled binning; false (nothing launched) when no kernel fits.
+template <typename T>
+bool launch_tile_grad(const Geom &ge, const Binning &b, const cplx_t<T> *g, const cplx_t<T> *v, T *gx,
+                      hipStream_t s);
 
 }  // namespace cunfft

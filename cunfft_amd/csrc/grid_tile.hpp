@@ -498,6 +498,77 @@ template <bool COMP, typename T> __device__ __forceinline__ void fold_add(T &s, 
     }
 }
 
+// Stages the halo-padded grid tile `tile_id` into LDS (rows of the pitch,
+// 16 bytes per load, every padded row wrapped onto the grid).  rowoff_mem is
+// LDS scratch for the padded rows' grid offsets (used when they fit in
+// rowoff_bytes, otherwise each offset is recomputed per load).  The caller
+// puts a barrier after.
+template <typename T, int D, typename L, int NT>
+__device__ __forceinline__ void stage_tile(const cplx_t<T> *__restrict__ g, uint32_t tile_id,
+                                           cplx_t<T> *tile, const Geom &ge, const BinGeom &bg,
+                                           void *rowoff_mem, size_t rowoff_bytes)
+{
+    using VT = vec16_t<T>;
+    constexpr int CPL = L::CPL;
+    // lane units staged per row: the whole pitch, spare units included, so the
+    // lanes of a wavefront write consecutive LDS units (no bank conflicts)
+    constexpr int NR = L::RP / CPL;
+    const int tid = threadIdx.x;
+    uint32_t rem = tile_id, o[3] = {0, 0, 0};
+    for (int t = D - 1; t >= 0; --t) {
+        o[t] = (rem % (uint32_t)bg.nt[t]) << bg.logT[t];
+        rem /= (uint32_t)bg.nt[t];
+    }
+    const uint32_t n0 = (uint32_t)ge.n[0], nL = (uint32_t)ge.n[D - 1];
+    const uint32_t n1 = D == 3 ? (uint32_t)ge.n[1] : 1;
+    const uint32_t s1 = D == 3 ? (uint32_t)ge.n[2] : 0;
+    const uint32_t s0 = D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : (uint32_t)ge.n[1];
+    VT *tv = reinterpret_cast<VT *>(tile);
+    const VT *gv = reinterpret_cast<const VT *>(g);
+    // A thread keeps one unit column un of the rows; its RS = NT / NR
+    // sibling threads per column take rows r0, r0 + RS, ... (NT mod NR
+    // threads idle), so the column's wrapped grid offset is computed once.
+    // 8 loads in flight per thread before the LDS stores (one at a time,
+    // each would wait out the full memory latency).
+    constexpr int ROWS = L::P0 * L::P1, RS = NT / NR, SB = 8;
+    static_assert(RS >= 1, "a workgroup covers a padded row");
+    // wrapped grid offset of a padded row; when they fit, all of them are
+    // computed once per workgroup into rowoff_mem (the caller's weight-batch
+    // memory, unused until the tile is staged)
+    auto row_offset = [&](int row) {
+        const int p0 = row / L::P1, p1 = row - p0 * L::P1;
+        // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
+        uint32_t off = wrap_hi32(o[0] + p0, n0) * s0;
+        if (D == 3) off += wrap_hi32(o[1] + p1, n1) * s1;
+        return off;
+    };
+    const bool TABLE = ROWS * sizeof(uint32_t) <= rowoff_bytes;
+    uint32_t *rowoff = static_cast<uint32_t *>(rowoff_mem);
+    if (TABLE) {
+        for (int row = tid; row < ROWS; row += NT) rowoff[row] = row_offset(row);
+        __syncthreads();
+    }
+    const int un = tid % NR, r0 = tid / NR;
+    const uint32_t colc = wrap_hi32(o[D - 1] + CPL * un, nL);
+    if (r0 < RS) {
+        for (int rb = r0; rb < ROWS; rb += SB * RS) {
+            VT v[SB];
+            int at[SB];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int row = rb + u * RS < ROWS ? rb + u * RS : ROWS - 1;
+                at[u] = (row * L::RP) / CPL + un;
+                v[u] = gv[((TABLE ? rowoff[row] : row_offset(row)) + colc) / CPL];
+            }
+            // unconditional stores (a thread past the last row rewrites it
+            // with its own value): a conditional store would pull its load
+            // back into the branch, one round trip at a time
+#pragma unroll
+            for (int u = 0; u < SB; ++u) tv[at[u]] = v[u];
+        }
+    }
+}
+
 // ------------------------------------------------------------ gather ----
 // NW wavefronts share the staged tile and take batches of the chunk's nodes
 // round-robin.  Each lane accumulates its unit over the node's planes and
@@ -516,9 +587,6 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
     using C = cplx_t<T>;
     constexpr int CPL = L::CPL;
     constexpr int NT = NW * 64;
-    // lane units staged per row: the whole pitch, spare units included, so the
-    // lanes of a wavefront write consecutive LDS units (no bank conflicts)
-    constexpr int NR = L::RP / CPL;
     constexpr int PB = L::PB;
     constexpr int NB = batch_nodes<T, D, W>(L::CELLS * L::CB, NW, 4 * (int)sizeof(C));
     __shared__ __align__(16) C tile[L::CELLS];
@@ -538,62 +606,7 @@ __device__ __forceinline__ void tile_gather(const cplx_t<T> *__restrict__ g,
         }
     }
 
-    // ---- stage the padded tile: rows of RL cells, 16 bytes per load
-    {
-        uint32_t rem = ch.tile, o[3] = {0, 0, 0};
-        for (int t = D - 1; t >= 0; --t) {
-            o[t] = (rem % (uint32_t)bg.nt[t]) << bg.logT[t];
-            rem /= (uint32_t)bg.nt[t];
-        }
-        const uint32_t n0 = (uint32_t)ge.n[0], nL = (uint32_t)ge.n[D - 1];
-        const uint32_t n1 = D == 3 ? (uint32_t)ge.n[1] : 1;
-        const uint32_t s1 = D == 3 ? (uint32_t)ge.n[2] : 0;
-        const uint32_t s0 = D == 3 ? (uint32_t)(ge.n[1] * ge.n[2]) : (uint32_t)ge.n[1];
-        VT *tv = reinterpret_cast<VT *>(tile);
-        const VT *gv = reinterpret_cast<const VT *>(g);
-        // A thread keeps one unit column un of the rows; its RS = NT / NR
-        // sibling threads per column take rows r0, r0 + RS, ... (NT mod NR
-        // threads idle), so the column's wrapped grid offset is computed once.
-        // 8 loads in flight per thread before the LDS stores (one at a time,
-        // each would wait out the full memory latency).
-        constexpr int ROWS = L::P0 * L::P1, RS = NT / NR, SB = 8;
-        static_assert(RS >= 1, "a workgroup covers a padded row");
-        // wrapped grid offset of a padded row; when they fit, all of them are
-        // computed once per workgroup into the weight-batch memory (unused
-        // until the tile is staged)
-        auto row_offset = [&](int row) {
-            const int p0 = row / L::P1, p1 = row - p0 * L::P1;
-            // o + p < 2n: the padded tile reaches at most W - 1 cells past the grid
-            uint32_t off = wrap_hi32(o[0] + p0, n0) * s0;
-            if (D == 3) off += wrap_hi32(o[1] + p1, n1) * s1;
-            return off;
-        };
-        constexpr bool TABLE = ROWS * sizeof(uint32_t) <= sizeof(wbs);
-        uint32_t *rowoff = reinterpret_cast<uint32_t *>(&wbs[0]);
-        if constexpr (TABLE) {
-            for (int row = tid; row < ROWS; row += NT) rowoff[row] = row_offset(row);
-            __syncthreads();
-        }
-        const int un = tid % NR, r0 = tid / NR;
-        const uint32_t colc = wrap_hi32(o[D - 1] + CPL * un, nL);
-        if (r0 < RS) {
-            for (int rb = r0; rb < ROWS; rb += SB * RS) {
-                VT v[SB];
-                int at[SB];
-#pragma unroll
-                for (int u = 0; u < SB; ++u) {
-                    const int row = rb + u * RS < ROWS ? rb + u * RS : ROWS - 1;
-                    at[u] = (row * L::RP) / CPL + un;
-                    v[u] = gv[((TABLE ? rowoff[row] : row_offset(row)) + colc) / CPL];
-                }
-                // unconditional stores (a thread past the last row rewrites it
-                // with its own value): a conditional store would pull its load
-                // back into the branch, one round trip at a time
-#pragma unroll
-                for (int u = 0; u < SB; ++u) tv[at[u]] = v[u];
-            }
-        }
-    }
+    stage_tile<T, D, L, NT>(g, ch.tile, tile, ge, bg, &wbs[0], sizeof(wbs));
     __syncthreads();
 
     LaneItem it[L::NI];
@@ -713,6 +726,240 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather_batch(const GridJob<T> 
     const GridJob<T> &J = jobs[bm.job];
     tile_gather<T, D, W, NW, V>(J.g, static_cast<const NodeDesc<T> *>(J.desc), J.perm,
                                 J.chunks[bm.blk], J.f_out, J.ge, J.bg, J.wc);
+}
+
+// --------------------------------------------------------- x-gradient ----
+// d f_j / d x_jt = sum_l g_l n_t phi'(s_t) prod_{t' != t} phi(s_t'): the
+// gather's tile and lane mapping with the derivative weights beside the
+// weights (the gradient of src/cunfft_kernel.cu:918-1061's sum with the
+// window index set held fixed).  Per lane and unit two plane sums, one with
+// the plane weights and one with their derivatives; the row and column
+// weights (or their derivatives) then give the D partial derivatives, summed
+// over the lanes like the gather's value.  Output gx[j d + t] =
+// Re(conj(v_j) d f_j / d x_jt) for the upstream v.
+template <typename T, int D, int W, int NB> struct alignas(16) DwBatch {
+    static constexpr int V0 = (W + 3) & ~3;
+    T d0[NB][V0];       // plane-weight derivatives
+    T d1[NB][W];        // row-weight derivatives (d = 3)
+    T d2[NB][W + 4];    // column-weight derivatives, tap l at l + 1, zero at 0 and past W
+};
+
+template <typename T> __device__ __noinline__ T alt_dphi_call(int win, T s, int m, T shape)
+{
+    return win == WIN_SINC ? sinc_dphi<T>(s, m, shape) : kb_dphi<T>(s, (T)m, shape);
+}
+
+// After batch_prologue: the weights' x-derivatives n_t phi'(s) of the batch
+// (Gaussian: -2 n_t s / b_t phi(s) from the weight; the others out of line).
+template <typename T, int D, int W, int NB>
+__device__ __forceinline__ void dw_prologue(DwBatch<T, D, W, NB> &db, const WinBatch<T, D, W, NB> &wb,
+                                            int nb, int lane, const WinConst<T> &wc)
+{
+    if (lane < nb) {
+        db.d2[lane][0] = (T)0;
+        db.d2[lane][W + 1] = (T)0;
+        db.d2[lane][W + 2] = (T)0;
+        db.d2[lane][W + 3] = (T)0;
+    }
+    constexpr int NPASS = (NB * W + 63) / 64;
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+#pragma unroll
+        for (int ps = 0; ps < NPASS; ++ps) {
+            const int it = lane + 64 * ps;
+            if (it >= nb * W) break;
+            const int n = it / W, l = it - n * W;
+            const T sv = wb.fx[n][t] + (T)(wc.m - l);
+            const T w = t == 0 ? wb.w0[n][l] : (t == D - 1 ? wb.w2[n][l + 1] : wb.w1[n][l]);
+            const T dw = wc.win != WIN_GAUSSIAN ? wc.nn[t] * alt_dphi_call<T>(wc.win, sv, wc.m, wc.shp[t])
+                                                : wc.dk[t] * sv * w;
+            if (t == 0) db.d0[n][l] = dw;
+            else if (t == D - 1) db.d2[n][l + 1] = dw;
+            else db.d1[n][l] = dw;
+        }
+    }
+    lds_order();
+}
+
+// Workgroup shape of the x-gradient: the gather's wavefronts, node batches
+// sized with the derivative weights and D partial sums per node.
+template <typename T, int D, int W, int V> struct GradShape {
+    using L = TileLayout<T, D, W, 1, V>;
+    static constexpr int NW = (2 * (L::CELLS * L::CB + 1024) > 160 * 1024) ? 8 : 4;
+    static constexpr int NB = batch_nodes<T, D, W>(
+        L::CELLS * L::CB, NW, 4 * D * (int)sizeof(cplx_t<T>) + (int)sizeof(DwBatch<T, D, W, 1>));
+    static constexpr long lds = (long)L::CELLS * L::CB +
+                                NW * ((long)sizeof(WinBatch<T, D, W, NB>) + (long)sizeof(DwBatch<T, D, W, NB>) +
+                                      (long)NB * 4 * D * (long)sizeof(cplx_t<T>));
+    static constexpr bool fits = L::fits && lds <= kLdsMax;
+};
+
+template <typename T, int D, int W, int NW, int V>
+__device__ __forceinline__ void tile_grad(const cplx_t<T> *__restrict__ g,
+                                          const NodeDesc<T> *__restrict__ desc,
+                                          const uint32_t *__restrict__ perm, const Chunk ch,
+                                          const cplx_t<T> *__restrict__ v, T *__restrict__ gx,
+                                          const Geom &ge, const BinGeom &bg, const WinConst<T> &wc)
+{
+    using L = TileLayout<T, D, W, 1, V>;
+    using VT = vec16_t<T>;
+    using C = cplx_t<T>;
+    constexpr int CPL = L::CPL;
+    constexpr int NT = NW * 64;
+    constexpr int PB = L::PB;
+    constexpr int NB = GradShape<T, D, W, V>::NB;
+    __shared__ __align__(16) C tile[L::CELLS];
+    __shared__ WinBatch<T, D, W, NB> wbs[NW];
+    __shared__ DwBatch<T, D, W, NB> dbs[NW];
+    __shared__ C red[NW][NB][4][D];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nbat = (ch.count + NB - 1) / NB;
+
+    NodeDesc<T> nd{};
+    uint32_t pm = 0;
+    {
+        const uint32_t s = (uint32_t)wave * NB + lane;
+        if (lane < NB && s < ch.count) {
+            nd = desc[ch.start + s];
+            pm = perm[ch.start + s];
+        }
+    }
+    stage_tile<T, D, L, NT>(g, ch.tile, tile, ge, bg, &wbs[0], sizeof(wbs));
+    __syncthreads();
+
+    LaneItem it[L::NI];
+#pragma unroll
+    for (int i = 0; i < L::NI; ++i) it[i] = lane_item<L>(i, lane);
+    const char *tb = reinterpret_cast<const char *>(tile);
+    WinBatch<T, D, W, NB> &wb = wbs[wave];
+    DwBatch<T, D, W, NB> &db = dbs[wave];
+
+    for (uint32_t bi = wave; bi < nbat; bi += NW) {
+        const uint32_t s0 = bi * NB;
+        const int nb = ch.count - s0 < (uint32_t)NB ? (int)(ch.count - s0) : NB;
+        const NodeRec rc = batch_prologue<T, D, W, NB, L>(wb, nb, lane, nd, wc);
+        dw_prologue<T, D, W, NB>(db, wb, nb, lane, wc);
+        const uint32_t pm_out = pm;
+        {
+            const uint32_t s = s0 + NW * NB + lane;
+            if (lane < NB && s < ch.count) {
+                nd = desc[ch.start + s];
+                pm = perm[ch.start + s];
+            }
+        }
+        for (int k = 0; k < nb; ++k) {
+            const int nbase = rlane_i(rc.nbase, k), par = rlane_i(rc.par, k);
+            T sx[D], sy[D];   // partial derivatives along each axis, this lane's share
+#pragma unroll
+            for (int t = 0; t < D; ++t) sx[t] = sy[t] = 0;
+#pragma unroll
+            for (int i = 0; i < L::NI; ++i) {
+                if (!it[i].ok) continue;
+                const uint32_t lb = (uint32_t)(nbase + it[i].off - it[i].q * L::PP) * L::CB;
+                const int cw = it[i].u2 - par + 1;
+                // plane sums of the unit's cells: with the weights (a) and the
+                // derivative weights (b); cells 0 and 1 of the unit
+                T a0x = 0, a0y = 0, a1x = 0, a1y = 0, b0x = 0, b0y = 0, b1x = 0, b1y = 0;
+                T w1v = 0, w2a = 0, w2b = 0, d1v = 0, d2a = 0, d2b = 0;
+#pragma unroll
+                for (int g0 = 0; g0 < L::NGRP; g0 += PB) {
+                    VT vv[PB];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int gq = g0 + j;
+                        if (gq >= L::NGRP) break;
+                        int a = gq * L::G + it[i].q;
+                        a = a < W ? a : W - 1;   // planes past the window: weights 0 below
+                        vv[j] = *reinterpret_cast<const VT *>(tb + lb + (uint32_t)(a * L::PP * L::CB));
+                    }
+                    lds_order();
+                    if (g0 == 0) {
+                        w1v = D == 3 ? wb.w1[k][it[i].r] : (T)1;
+                        d1v = D == 3 ? db.d1[k][it[i].r] : (T)0;
+                        w2a = wb.w2[k][cw];
+                        d2a = db.d2[k][cw];
+                        w2b = CPL == 2 ? wb.w2[k][cw + 1] : (T)0;
+                        d2b = CPL == 2 ? db.d2[k][cw + 1] : (T)0;
+                    }
+                    T w0r[PB], d0r[PB];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int a = (g0 + j) * L::G + it[i].q;
+                        w0r[j] = wb.w0[k][a < W ? a : W - 1];
+                        d0r[j] = db.d0[k][a < W ? a : W - 1];
+                    }
+                    lds_order();
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const int gq = g0 + j;
+                        if (gq >= L::NGRP) break;
+                        const int a = gq * L::G + it[i].q;
+                        T w0a = w0r[j], d0a = d0r[j];
+                        if constexpr (L::G > 1) {
+                            w0a = a < W ? w0a : (T)0;
+                            d0a = a < W ? d0a : (T)0;
+                        }
+                        if constexpr (CPL == 2) {
+                            a0x += w0a * vv[j].x;
+                            a0y += w0a * vv[j].y;
+                            a1x += w0a * vv[j].z;
+                            a1y += w0a * vv[j].w;
+                            b0x += d0a * vv[j].x;
+                            b0y += d0a * vv[j].y;
+                            b1x += d0a * vv[j].z;
+                            b1y += d0a * vv[j].w;
+                        } else {
+                            a0x += w0a * vv[j].x;
+                            a0y += w0a * vv[j].y;
+                            b0x += d0a * vv[j].x;
+                            b0y += d0a * vv[j].y;
+                        }
+                    }
+                }
+                // axis 0: plane derivative, row and column weights
+                const T wu = w1v * w2a, wv = w1v * w2b;
+                sx[0] += wu * b0x + wv * b1x;
+                sy[0] += wu * b0y + wv * b1y;
+                if constexpr (D == 3) {   // axis 1: row derivative
+                    const T ru = d1v * w2a, rv = d1v * w2b;
+                    sx[1] += ru * a0x + rv * a1x;
+                    sy[1] += ru * a0y + rv * a1y;
+                }
+                // last axis: column derivative
+                const T cu = w1v * d2a, cv = w1v * d2b;
+                sx[D - 1] += cu * a0x + cv * a1x;
+                sy[D - 1] += cu * a0y + cv * a1y;
+            }
+#pragma unroll
+            for (int t = 0; t < D; ++t) {
+                const T rx = row_sum15<T>(sx[t]), ry = row_sum15<T>(sy[t]);
+                if ((lane & 15) == 15) red[wave][k][lane >> 4][t] = cmake<T>(rx, ry);
+            }
+        }
+        lds_order();
+        if (lane < nb) {
+            const C vj = v[pm_out];
+#pragma unroll
+            for (int t = 0; t < D; ++t) {
+                const C r0 = red[wave][lane][0][t], r1 = red[wave][lane][1][t];
+                const C r2 = red[wave][lane][2][t], r3 = red[wave][lane][3][t];
+                const T Sx = (r0.x + r1.x) + (r2.x + r3.x), Sy = (r0.y + r1.y) + (r2.y + r3.y);
+                gx[(int64_t)pm_out * D + t] = vj.x * Sx + vj.y * Sy;
+            }
+        }
+        lds_order();
+    }
+}
+
+template <typename T, int D, int W, int NW, int V>
+__global__ __launch_bounds__(NW * 64) void k_tile_grad(
+    const cplx_t<T> *__restrict__ g, const NodeDesc<T> *__restrict__ desc,
+    const uint32_t *__restrict__ perm, const Chunk *__restrict__ chunks,
+    const uint32_t *__restrict__ n_chunks, const cplx_t<T> *__restrict__ v, T *__restrict__ gx, Geom ge,
+    BinGeom bg, WinConst<T> wc)
+{
+    if (blockIdx.x >= n_chunks[0]) return;
+    tile_grad<T, D, W, NW, V>(g, desc, perm, chunks[blockIdx.x], v, gx, ge, bg, wc);
 }
 
 // ------------------------------------------------------------ spread ----
@@ -1547,6 +1794,8 @@ template <typename T> WinConst<T> win_const(const Geom &ge)
         w.kb[t] = (T)(ge.invb[t] * sc);
         w.nrm[t] = (T)ge.nrm[t];
         w.shp[t] = (T)ge.shp[t];
+        w.dk[t] = (T)(-2.0 * (double)ge.n[t] * ge.invb[t]);
+        w.nn[t] = (T)ge.n[t];
     }
     w.m = ge.m;
     w.win = ge.win;
@@ -1640,6 +1889,27 @@ template <typename T> struct GatherOp {
             k_tile_gather<T, D, W, NWG, V><<<(unsigned)b.max_chunks, NWG * 64, 0, s>>>(
                     g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, f,
                     ge, bin_geom(b), win_const<T>(ge));
+            return true;
+        } else {
+            return false;
+        }
+    }
+};
+
+template <typename T> struct GradOp {
+    const Geom &ge;
+    const Binning &b;
+    const cplx_t<T> *g;
+    const cplx_t<T> *v;
+    T *gx;
+    hipStream_t s;
+    template <int D, int W, int V> bool run()
+    {
+        if constexpr (fits_all<T, D, W, V>() && GradShape<T, D, W, V>::fits) {
+            constexpr int NWG = GradShape<T, D, W, V>::NW;
+            k_tile_grad<T, D, W, NWG, V><<<(unsigned)b.max_chunks, NWG * 64, 0, s>>>(
+                g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, v, gx, ge,
+                bin_geom(b), win_const<T>(ge));
             return true;
         } else {
             return false;
@@ -1743,6 +2013,16 @@ void launch_tile_gather(const Geom &ge, const Binning &b, const cplx_t<T> *g, cp
     if (!with_dwv(ge.d, ge.W, b.tile_variant, op))
         throw Error(CUNFFT_ERR_UNSUPPORTED, "no tiled gather for this geometry");
     CUNFFT_HIP(hipGetLastError());
+}
+
+template <typename T>
+bool launch_tile_grad(const Geom &ge, const Binning &b, const cplx_t<T> *g, const cplx_t<T> *v, T *gx,
+                      hipStream_t s)
+{
+    GradOp<T> op{ge, b, g, v, gx, s};
+    if (!with_dwv(ge.d, ge.W, b.tile_variant, op)) return false;
+    CUNFFT_HIP(hipGetLastError());
+    return true;
 }
 
 template <typename T>

@@ -11,6 +11,8 @@ template void launch_tile_gather<float>(const Geom &, const Binning &, const flo
 template void launch_tile_spread<float>(const Geom &, const Binning &, const float2 *, float2 *,
                                        hipStream_t, hipEvent_t);
 template WinConst<float> win_const<float>(const Geom &);
+template bool launch_tile_grad<float>(const Geom &, const Binning &, const float2 *, const float2 *, float *,
+                                     hipStream_t);
 template void launch_tile_batch<float>(BatchStage, const GridJob<float> *, const BlockMap *, uint32_t,
                                       int, int, int, hipStream_t);
 

@@ -11,6 +11,8 @@ template void launch_tile_gather<double>(const Geom &, const Binning &, const do
 template void launch_tile_spread<double>(const Geom &, const Binning &, const double2 *, double2 *,
                                        hipStream_t, hipEvent_t);
 template WinConst<double> win_const<double>(const Geom &);
+template bool launch_tile_grad<double>(const Geom &, const Binning &, const double2 *, const double2 *, double *,
+                                     hipStream_t);
 template void launch_tile_batch<double>(BatchStage, const GridJob<double> *, const BlockMap *, uint32_t,
                                       int, int, int, hipStream_t);
 

@@ -536,6 +536,21 @@ static const Binning &ensure_bins(Impl<T> *im, int dir, const double *x, hipStre
     return im->bins[k];
 }
 
+// x-gradient gx[j d + t] = Re(conj(v_j) d f_j / d x_jt) over the forward grid g:
+// the binned plane-tiled kernel on the forward configuration's binning, else
+// the node-parallel kernel (1D, node lanes, the plain gather, or no fitting
+// tile).
+template <typename T>
+static void grad_x(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T> *g, const cplx_t<T> *v,
+                   T *gx, hipStream_t s)
+{
+    if (!(p->flags & CUNFFT_PLAIN_GATHER)) {
+        const Binning &b = ensure_bins<T>(im, 0, p->x_gpu, s);
+        if (b.usable && b.built && b.planes() && launch_tile_grad<T>(im->geom, b, g, v, gx, s)) return;
+    }
+    launch_grad_x<T>(im->geom, g, p->x_gpu, v, gx, s);
+}
+
 // f = A fhat.  Returns the event-timed (fft, conv) seconds when timing.
 template <typename T>
 static void run_forward(typename Traits<T>::plan *p, Impl<T> *im, const cplx_t<T> *fhat,
@@ -1048,7 +1063,7 @@ int api_grad_x_dev(typename Traits<T>::plan *p, const void *fhat, const void *v,
         const cplx_t<T> *g = fft_passes<T>(p, im, fft_tables<T>(p, true), true,
                                            static_cast<const cplx_t<T> *>(fhat), nullptr,
                                            reinterpret_cast<cplx_t<T> *>(p->g_gpu), s);
-        launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(v), gx, s);
+        grad_x<T>(p, im, g, static_cast<const cplx_t<T> *>(v), gx, s);
     });
 }
 
@@ -1071,7 +1086,7 @@ int api_backward_dev(typename Traits<T>::plan *p, const void *fhat, const void *
             const cplx_t<T> *g = fft_passes<T>(p, im, fft_tables<T>(p, true), true,
                                                static_cast<const cplx_t<T> *>(fhat), nullptr,
                                                reinterpret_cast<cplx_t<T> *>(p->g_gpu), s);
-            launch_grad_x<T>(im->geom, g, p->x_gpu, static_cast<const cplx_t<T> *>(gf), gx, s);
+            grad_x<T>(p, im, g, static_cast<const cplx_t<T> *>(gf), gx, s);
         }
         if (gfhat) {
             double a, b, c;

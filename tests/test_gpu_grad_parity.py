@@ -7,8 +7,9 @@ src/cunfft_kernel.cu:514-1093 with the window index set held fixed, with the
 weights recomputed (nothing stored from the forward).  The oracle's
 nfft_dx / nfft_adjoint give the same quantities in FP64: FP64 plans match to
 1e-12, FP32 plans to 1e-5 on FP32-rounded inputs.  Every gridding
-configuration of the plan is run (the x-gradient follows the forward's
-binning).
+configuration of the plan is run: for d >= 2 the plane-tiled configurations
+run the binned derivative gather (k_tile_grad) on the forward's binning, the
+node-lane configuration and d = 1 the node-parallel k_grad_x.
 """
 import numpy as np
 import pytest

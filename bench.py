@@ -115,11 +115,39 @@ def time_plan(cu, precision, x, fh, fv, steps, warmup):
         rec["adj_fft"].append(t["time_FFT"])
     p._p.flags = p._p.flags & ~cu.MEASURE_TIME
     med = {k: 1e3 * float(np.median(v)) for k, v in rec.items()}
+    bwd = time_backward(p, precision, steps)
     return p, {"config": list(config), "autotune_fwd_ms": ms_f, "autotune_adj_ms": ms_a,
                "ms_per_step": ms_per_step, "cold_binning_ms_per_step": cold_ms,
                "fwd_fft_ms": med["fwd_fft"], "fwd_gather_ms": med["fwd_gather"],
                "adj_scatter_ms": med["adj_grid"] - med["adj_fold"], "adj_fold_ms": med["adj_fold"],
-               "adj_fft_ms": med["adj_fft"]}
+               "adj_fft_ms": med["adj_fft"], **bwd}
+
+
+def time_backward(p, precision, steps):
+    """Backward of f = A(x) fhat on the same plan (binning warm): the full
+    backward (grad_x from the forward grid of fhat -- roll-off, FFT, derivative
+    gather -- then grad_fhat = A^H grad_f, one adjoint NFFT) and the
+    x-gradient alone, with grad_f = the plan's f."""
+    import torch
+
+    dp = p.device_pointers
+    real = torch.float32 if precision == "single" else torch.float64
+    cplx = torch.complex64 if precision == "single" else torch.complex128
+    gx = torch.empty(M_NODES * 3, dtype=real, device="cuda")
+    gfh = torch.empty(int(np.prod(N3)), dtype=cplx, device="cuda")
+    reps = max(3, min(steps, 10))
+    out = {}
+    for key, call in (("backward_ms", lambda: p.backward_dev(dp["f_hat"], dp["f"], gfh.data_ptr(),
+                                                             gx.data_ptr())),
+                      ("grad_x_ms", lambda: p.grad_x_dev(dp["f_hat"], dp["f"], gx.data_ptr()))):
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        torch.cuda.synchronize()
+        out[key] = 1e3 * (time.perf_counter() - t0) / reps
+    return out
 
 
 def pcie_inclusive(cu, x, fh, fv, steps=4):
@@ -274,6 +302,8 @@ def cpu_baseline(x, fh, budget_s=15.0):
             "sample": f"FP64 direct NDFT (oracle, OpenMP) of {n_rows} sampled output rows of the "
                       f"forward transform ({terms:.3g} terms in {elapsed:.2f} s); fwd+adj rate "
                       f"extrapolated linearly to the full 2*M*N_total terms",
+            "cores_note": "OpenMP threads used: the process's CPU affinity, capped at 16 "
+                          "(the host share of one GPU)",
             "value_1_thread": value1, "sample_1_thread_rows": int(len(rows1)),
             "cpu_model": model, "nproc": os.cpu_count()}
 
@@ -527,6 +557,10 @@ def main():
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": measured_traffic(kern) if (cf if kern == "fwd_gather_ms" else ca) == 0
             else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of this kernel in this "
+                              "configuration, scaled by a plain device copy's counters "
+                              "(profiles/traffic_gridding_3d128_fp32.txt); null when that "
+                              "summary does not cover the configuration the autotuner chose",
             "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
     cpu = None
     if not a.no_cpu_baseline:

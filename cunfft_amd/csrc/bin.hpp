@@ -41,6 +41,32 @@ struct Chunk {
     uint32_t pad;
 };
 
+// Workgroups of a grid are dispatched round-robin over the 8 XCDs of an
+// MI355X (workgroup b runs on XCD b mod 8), each XCD with its own L2.  The
+// tile-staging kernels deal the chunks (tiles in row-major order) to the
+// XCDs in groups of kXcdGroup consecutive chunks, group g to XCD g mod 8:
+// within a group, neighbouring tiles, whose halo-padded regions overlap, and
+// the chunks of one dense tile are staged through the same L2, while the
+// groups of a dense region still spread over all XCDs (contiguous eighths
+// would put a node cluster on one or two XCDs).  Chunk of workgroup b, or -1
+// when b maps past the n chunks; the grid must hold xcd_grid(max_chunks)
+// workgroups for any n <= max_chunks (workgroup b can map up to
+// 7 (kXcdGroup - 1) chunks below b).
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdGroup = 128;
+__host__ __device__ __forceinline__ int64_t xcd_chunk(uint32_t b, uint32_t n)
+{
+    const uint32_t x = b % kXcds, local = b / kXcds;
+    const uint64_t g = (uint64_t)(local / kXcdGroup) * kXcds + x;
+    const uint64_t c = g * kXcdGroup + local % kXcdGroup;
+    return c < n ? (int64_t)c : -1;
+}
+inline unsigned xcd_grid(int64_t max_chunks)
+{
+    const int64_t need = max_chunks + (int64_t)(kXcds - 1) * (kXcdGroup - 1);
+    return (unsigned)((need + kXcds - 1) / kXcds * kXcds);
+}
+
 // Per-node window parameters in sorted order, written at binning time so the
 // gridding kernels do no FP64 index arithmetic:
 //   f[t]  = c_t - m, with c_t = n_t x_t - u_t the offset of the node from its

@@ -713,8 +713,9 @@ __global__ __launch_bounds__(NW * 64) void k_tile_gather(
     const uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ f, Geom ge, BinGeom bg,
     WinConst<T> wc)
 {
-    if (blockIdx.x >= n_chunks[0]) return;
-    tile_gather<T, D, W, NW, V>(g, desc, perm, chunks[blockIdx.x], f, ge, bg, wc);
+    const int64_t c = xcd_chunk(blockIdx.x, n_chunks[0]);
+    if (c < 0) return;
+    tile_gather<T, D, W, NW, V>(g, desc, perm, chunks[c], f, ge, bg, wc);
 }
 
 // The gather of every chunk of every job of a batch in one grid.
@@ -958,8 +959,9 @@ __global__ __launch_bounds__(NW * 64) void k_tile_grad(
     const uint32_t *__restrict__ n_chunks, const cplx_t<T> *__restrict__ v, T *__restrict__ gx, Geom ge,
     BinGeom bg, WinConst<T> wc)
 {
-    if (blockIdx.x >= n_chunks[0]) return;
-    tile_grad<T, D, W, NW, V>(g, desc, perm, chunks[blockIdx.x], v, gx, ge, bg, wc);
+    const int64_t c = xcd_chunk(blockIdx.x, n_chunks[0]);
+    if (c < 0) return;
+    tile_grad<T, D, W, NW, V>(g, desc, perm, chunks[c], v, gx, ge, bg, wc);
 }
 
 // ------------------------------------------------------------ spread ----
@@ -1886,7 +1888,7 @@ template <typename T> struct GatherOp {
     {
         if constexpr (fits_all<T, D, W, V>()) {
             constexpr int NWG = Waves<T, D, W, V>::gather;
-            k_tile_gather<T, D, W, NWG, V><<<(unsigned)b.max_chunks, NWG * 64, 0, s>>>(
+            k_tile_gather<T, D, W, NWG, V><<<xcd_grid(b.max_chunks), NWG * 64, 0, s>>>(
                     g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, f,
                     ge, bin_geom(b), win_const<T>(ge));
             return true;
@@ -1907,7 +1909,7 @@ template <typename T> struct GradOp {
     {
         if constexpr (fits_all<T, D, W, V>() && GradShape<T, D, W, V>::fits) {
             constexpr int NWG = GradShape<T, D, W, V>::NW;
-            k_tile_grad<T, D, W, NWG, V><<<(unsigned)b.max_chunks, NWG * 64, 0, s>>>(
+            k_tile_grad<T, D, W, NWG, V><<<xcd_grid(b.max_chunks), NWG * 64, 0, s>>>(
                 g, static_cast<const NodeDesc<T> *>(b.desc), b.perm, b.chunks, b.n_chunks, v, gx, ge,
                 bin_geom(b), win_const<T>(ge));
             return true;

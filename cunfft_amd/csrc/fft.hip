@@ -172,6 +172,108 @@ __device__ __forceinline__ void stage2_last(cplx_t<T> *buf, const cplx_t<T> *__r
     __syncthreads();
 }
 
+// 16-point DFT in registers with kernel exp(sign 2 pi i r q / 16), outputs in
+// natural order: n = 4 n1 + n2, k = k1 + 4 k2 gives four 4-point DFTs over
+// n1, the twiddles exp(sign 2 pi i n2 k1 / 16), and four 4-point DFTs over n2.
+template <typename T> __device__ __forceinline__ void dft16(cplx_t<T> (&v)[16], int sign)
+{
+    const T c1 = (T)0.92387953251128675613;   // cos(pi/8)
+    const T s1 = (T)0.38268343236508977173;   // sin(pi/8)
+    const T h = (T)0.70710678118654752440;    // sqrt(1/2)
+    const T sg = (T)sign;
+    cplx_t<T> y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cplx_t<T> a[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft4<T>(a, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = a[k1];
+    }
+    // z exp(sign i theta) for cos theta = c, sin theta = s
+    auto rot = [&](cplx_t<T> z, T c, T s) {
+        return cmake<T>(z.x * c - z.y * (sg * s), z.x * (sg * s) + z.y * c);
+    };
+    y[1][1] = rot(y[1][1], c1, s1);    // pi/8
+    y[1][2] = rot(y[1][2], h, h);      // pi/4
+    y[1][3] = rot(y[1][3], s1, c1);    // 3pi/8
+    y[2][1] = rot(y[2][1], h, h);      // pi/4
+    y[2][2] = cmake<T>(-sg * y[2][2].y, sg * y[2][2].x);   // pi/2
+    y[2][3] = rot(y[2][3], -h, h);     // 3pi/4
+    y[3][1] = rot(y[3][1], s1, c1);    // 3pi/8
+    y[3][2] = rot(y[3][2], -h, h);     // 3pi/4
+    y[3][3] = rot(y[3][3], -c1, -s1);  // 9pi/8
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cplx_t<T> b[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft4<T>(b, sign);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+}
+
+// One radix-16 Stockham stage (power-of-two L, Ns = 2^logNs): butterfly j of
+// line c reads j + r Q (Q = L / 16), multiplies input r by
+// exp(sign 2 pi i r (j mod Ns) / (16 Ns)), and writes its 16-point DFT to
+// (j div Ns) 16 Ns + (j mod Ns) + r Ns.  A layout flag selects, per side, the
+// lines at pitch ldsL (element i at i) or the padded lines at pitch ldsP
+// (element i at i + i / 16): the first stage (Ns = 1) writes with stride 16,
+// which the padding spreads over all LDS banks.
+template <typename T, int E, bool PIN, bool POUT>
+__device__ __forceinline__ void stage16(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int logL,
+                                        int C, int ldsL, int ldsP, int logNs, int sign, int tid)
+{
+    const int logQ = logL - 4;
+    const int Q = 1 << logQ;
+    const int Ns = 1 << logNs;
+    const int nb = C << logQ;
+    const int pin = PIN ? ldsP : ldsL, pout = POUT ? ldsP : ldsL;
+    constexpr int KB = (E / 16 + kFftThreads - 1) / kFftThreads;
+    cplx_t<T> v[KB][16];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            const int c = b >> logQ, j = b & (Q - 1);
+            const cplx_t<T> *base = buf + c * pin;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = j + r * Q;
+                v[k][r] = base[PIN ? i + (i >> 4) : i];
+            }
+            if (Ns > 1) {
+                const int step = (j & (Ns - 1)) << (logL - 4 - logNs);
+#pragma unroll
+                for (int r = 1; r < 16; ++r) v[k][r] = cmul<T>(v[k][r], twiddle<T>(tw, r * step, sign));
+            }
+            dft16<T>(v[k], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (b < nb) {
+            const int c = b >> logQ, j = b & (Q - 1);
+            const int out = ((j >> logNs) << (logNs + 4)) + (j & (Ns - 1));
+            cplx_t<T> *ob = buf + c * pout;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = out + r * Ns;
+                ob[POUT ? i + (i >> 4) : i] = v[k][r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Lines of at least 256 points run two radix-16 stages first, the first
+// writing the padded layout; LDS elements per line of the widest layout.
+__host__ __device__ constexpr bool fft_radix16(int logL) { return logL >= 8; }
+__host__ __device__ constexpr int fft_lds_pitch(int L, int logL, int ldsL)
+{
+    return fft_radix16(logL) && L + (L >> 4) > ldsL ? L + (L >> 4) : ldsL;
+}
+
 // a = q d + r for 0 <= a < 2^22 with rd = 1/d in FP32 (exact after one
 // correction step)
 __device__ __forceinline__ void divmod_small(int a, int d, float rd, int &q, int &r)
@@ -298,7 +400,8 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
     }
     if (SPLIT == SPLIT_A && tid < C) s_i1[tid] = (int)((s0 + tid) / P.So);
     // the twiddle table of the axis, staged in LDS after the lines
-    C2 *twl = buf + C * ldsL;
+    const int ldsP = L + (L >> 4);   // padded line pitch of the radix-16 stages
+    C2 *twl = buf + C * fft_lds_pitch(L, logL, ldsL);
     for (int q = tid; q < L; q += kFftThreads) twl[q] = P.tw[q];
     __syncthreads();
 
@@ -391,6 +494,15 @@ __device__ __forceinline__ void fft_pass(const FftPass<T> &P, int64_t blk)
 
     if (pow2) {
         int logNs = 0;
+        if (fft_radix16(logL)) {
+            stage16<T, E, false, true>(buf, twl, logL, C, ldsL, ldsP, 0, P.sign, tid);
+            stage16<T, E, true, false>(buf, twl, logL, C, ldsL, ldsP, 4, P.sign, tid);
+            logNs = 8;
+            while (logNs + 4 <= logL) {
+                stage16<T, E, false, false>(buf, twl, logL, C, ldsL, ldsP, logNs, P.sign, tid);
+                logNs += 4;
+            }
+        }
         while (logNs + 2 <= logL) {
             stage4<T, E>(buf, twl, logL, C, ldsL, logNs, P.sign, tid);
             logNs += 2;
@@ -550,7 +662,8 @@ template <typename T> static void pass_shape(FftLaunch<T> &out)
     }
     P.C = (int)C;
     P.logC = ilog2_i(C);
-    out.lds = ((size_t)C * P.ldsL + P.L) * sizeof(cplx_t<T>);   // lines + twiddles
+    // lines (at the widest pitch the stages use) + twiddles
+    out.lds = ((size_t)C * fft_lds_pitch(P.L, P.logL, P.ldsL) + P.L) * sizeof(cplx_t<T>);
 }
 
 template <typename T>

@@ -4,11 +4,11 @@
 // forward through a texture copy of the grid (src/cunfft_kernel.cu:514-1093,
 // 1408-1418), the adjoint with two CAS-loop atomics per tap into global memory
 // (src/cunfft_kernel.cu:97-130, 608-1197).  Here nodes are first sorted on the
-// device by the grid tile holding their window start (counting sort: keys,
-// histogram, scan, placement), and each workgroup then owns one tile.  For
-// d >= 2 the plane-tiled kernels of grid_tile.hip use the sorted order; this
-// file holds the binning itself and the node-per-lane tiled kernels used for
-// d = 1 and under CUNFFT_NODE_LANES:
+// device by the grid tile holding their window start (a stable radix sort of
+// composite keys, then tile counts, scans and chunks), and each workgroup then
+// owns one tile.  For d >= 2 the plane-tiled kernels of grid_tile.hpp use the
+// sorted order; this file holds the binning itself and the node-per-lane
+// tiled kernels used for d = 1 and under CUNFFT_NODE_LANES:
 //   forward  the padded tile (T_t + 2m + 1 cells per axis) is staged in LDS
 //            once, and every node of the tile reads its (2m+2)^d taps there;
 //   adjoint  the node contributions accumulate in an LDS copy of the padded

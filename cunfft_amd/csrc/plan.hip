@@ -423,6 +423,10 @@ template <typename T> static void init_help(typename Traits<T>::plan *p)
     const int c0 = (p->flags & CUNFFT_NODE_LANES) && d >= 2 ? kTileVariants : 0;
     im->compensated = (p->flags & CUNFFT_COMPENSATED) && sizeof(T) == 4;
     bool tiled = plan_configs<T>(im, c0, c0);
+    if (im->compensated && (p->flags & CUNFFT_ATOMIC_SCATTER))
+        throw Error(CUNFFT_ERR_UNSUPPORTED,
+                    "CUNFFT_COMPENSATED needs the tiled adjoint; CUNFFT_ATOMIC_SCATTER adds with "
+                    "global atomics");
     if (im->compensated) {
         // the compensated sums live in the tiled adjoint kernels only (the
         // untiled fallback adds with global atomics): take the first tile

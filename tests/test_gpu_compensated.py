@@ -88,6 +88,11 @@ def test_compensated_plan_without_a_fitting_configuration_is_refused(gpu):
         gpu.Plan([32, 32], 1000, m=4, precision="single",
                  flags=gpu.CUFFT_INIT | gpu.CUFFT_BOTH | gpu.COMPENSATED | gpu.NODE_LANES)
     assert e.value.status == 2
+    # ... and with the atomic scatter asked for
+    with pytest.raises(gpu.CunfftError) as e:
+        gpu.Plan([32, 32], 1000, m=4, precision="single",
+                 flags=gpu.CUFFT_INIT | gpu.CUFFT_BOTH | gpu.COMPENSATED | gpu.ATOMIC_SCATTER)
+    assert e.value.status == 2
 
 
 def test_compensated_plan_refuses_the_node_lane_configuration(gpu):

@@ -1634,7 +1634,12 @@ __device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slo
     static_assert(L::T0 % 2 == 0 && (D == 2 || L::T1 % 2 == 0) && TL % 2 == 0,
                   "even tile extents keep slot rows and grid rows 16-byte aligned");
     const int tid = threadIdx.x;
-    const uint32_t tile = F::TPT <= 256 ? blk * F::TPW + (uint32_t)(tid / F::TPT) : blk / F::WPT;
+    // a tile's threads fill whole wavefronts (TPT >= 64), so the tile and every
+    // candidate-table entry below are wave-uniform: scalar loads, all issued
+    // before the first slot load
+    static_assert(F::TPT >= 64, "a wavefront never spans two tiles");
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(
+        F::TPT <= 256 ? blk * F::TPW + (uint32_t)(tid / F::TPT) : blk / F::WPT);
     if (tile >= (uint32_t)bg.ntiles) return;   // no barriers follow
     const int tt = F::TPT <= 256 ? tid % F::TPT : (int)(blk % F::WPT) * 256 + tid;
     const int col = tt / S, seg = tt % S;
@@ -1653,6 +1658,20 @@ __device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slo
     }
     const uint32_t n0 = (uint32_t)bg.nt[0], n1 = D == 3 ? (uint32_t)bg.nt[1] : 1u,
                    n2 = (uint32_t)bg.nt[D - 1];
+    constexpr int Q0 = L::Q0, Q1 = L::Q1, QL = L::QL;
+    // first chunk and chunk count of the candidate tile q back along each axis
+    uint32_t c0[Q0][Q1][QL], nch[Q0][Q1][QL];
+#pragma unroll
+    for (int q0 = 0; q0 < Q0; ++q0)
+#pragma unroll
+        for (int q1 = 0; q1 < Q1; ++q1)
+#pragma unroll
+            for (int qL = 0; qL < QL; ++qL) {
+                const uint32_t src = ((tk[0] + 4 * n0 - q0) % n0 * n1 + (tk[1] + 4 * n1 - q1) % n1) * n2 +
+                                     (tk[2] + 4 * n2 - qL) % n2;
+                c0[q0][q1][qL] = chunk_start[src];
+                nch[q0][q1][qL] = (tile_count[src] + kChunk - 1) / kChunk;
+            }
     T sx[SEGC], sy[SEGC], cx[SEGC], cy[SEGC];
 #pragma unroll
     for (int i = 0; i < SEGC; ++i) sx[i] = sy[i] = cx[i] = cy[i] = 0;
@@ -1670,66 +1689,63 @@ __device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slo
         }
     };
 #pragma unroll
-    for (int q0 = 0; q0 < L::Q0; ++q0)
+    for (int q0 = 0; q0 < Q0; ++q0) {
+        // the first chunk of every candidate of this q0 in flight together
+        C v[Q1][QL][SEGC];
+        bool in01[Q1];
+        int rc[Q1];
 #pragma unroll
-        for (int q1 = 0; q1 < L::Q1; ++q1) {
-            constexpr int QL = L::QL;
+        for (int q1 = 0; q1 < Q1; ++q1) {
             const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1);
-            const bool in01 = r0 < E0 && r1 < E1;
+            in01[q1] = r0 < E0 && r1 < E1;
+            rc[q1] = (r0 < E0 ? r0 : 0) * E1 + (r1 < E1 ? r1 : 0);
             // a halo block pair that covers none of the wavefront's columns
             // (E0 = 1 past the first rows of the tile) is not loaded at all
-            if (!__any(in01)) continue;
-            const int rc = ((r0 < E0 ? r0 : 0) * E1 + (r1 < E1 ? r1 : 0));
-            uint32_t c0[QL], nch[QL];
-            bool ok[QL];
-            C v[QL][SEGC];
-#pragma unroll
-            for (int qL = 0; qL < QL; ++qL) {
-                const uint32_t src = ((tk[0] + 4 * n0 - q0) % n0 * n1 + (tk[1] + 4 * n1 - q1) % n1) * n2 +
-                                     (tk[2] + 4 * n2 - qL) % n2;
-                const uint32_t cnt = tile_count[src];
-                c0[qL] = chunk_start[src];
-                nch[qL] = (cnt + kChunk - 1) / kChunk;
-                ok[qL] = cnt > 0 && in01;
-            }
+            if (!__any(in01[q1])) continue;
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
                 // a candidate with no nodes reads chunk 0 and adds nothing
-                const C *sp = slots + (int64_t)(nch[qL] > 0 ? c0[qL] : 0) * L::SC;
+                const C *sp = slots + (int64_t)(nch[q0][q1][qL] > 0 ? c0[q0][q1][qL] : 0) * L::SC;
                 const int EL = L::ext(TL, L::PL, qL);
-                const int o = L::block_off(q0, q1, qL) + rc * EL;
-                if (EL == TL) load_full(sp, o + seg * SEGC, v[qL]);
-                else load_last(sp, o, v[qL]);
+                const int o = L::block_off(q0, q1, qL) + rc[q1] * EL;
+                if (EL == TL) load_full(sp, o + seg * SEGC, v[q1][qL]);
+                else load_last(sp, o, v[q1][qL]);
             }
+        }
+#pragma unroll
+        for (int q1 = 0; q1 < Q1; ++q1) {
+            if (!__any(in01[q1])) continue;
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
                 const int EL = L::ext(TL, L::PL, qL);
+                const bool ok = nch[q0][q1][qL] > 0 && in01[q1];
 #pragma unroll
                 for (int i = 0; i < SEGC; ++i)
-                    if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && ok[qL]) {
-                        fold_add<COMP>(sx[i], cx[i], v[qL][i].x);
-                        fold_add<COMP>(sy[i], cy[i], v[qL][i].y);
+                    if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && ok) {
+                        fold_add<COMP>(sx[i], cx[i], v[q1][qL][i].x);
+                        fold_add<COMP>(sy[i], cy[i], v[q1][qL][i].y);
                     }
             }
             // further chunks of dense candidate tiles
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
                 const int EL = L::ext(TL, L::PL, qL);
-                const int o = L::block_off(q0, q1, qL) + rc * EL;
-                for (uint32_t c = 1; c < nch[qL]; ++c) {
-                    const C *sp = slots + (int64_t)(c0[qL] + c) * L::SC;
+                const int o = L::block_off(q0, q1, qL) + rc[q1] * EL;
+                for (uint32_t c = 1; c < nch[q0][q1][qL]; ++c) {
+                    const C *sp = slots + (int64_t)(c0[q0][q1][qL] + c) * L::SC;
                     C w[SEGC];
                     if (EL == TL) load_full(sp, o + seg * SEGC, w);
                     else load_last(sp, o, w);
 #pragma unroll
                     for (int i = 0; i < SEGC; ++i)
-                        if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && in01) {
+                        if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && in01[q1]) {
                             fold_add<COMP>(sx[i], cx[i], w[i].x);
                             fold_add<COMP>(sy[i], cy[i], w[i].y);
                         }
                 }
             }
         }
+    }
     const uint32_t g1 = D == 3 ? (uint32_t)ge.n[1] : 1u, gL = (uint32_t)ge.n[D - 1];
     C *dst = g + (size_t)((tk[0] * L::T0 + r0) * g1 + tk[1] * L::T1 + r1) * gL + tk[2] * TL + seg * SEGC;
 #pragma unroll

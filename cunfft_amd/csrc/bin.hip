@@ -63,16 +63,16 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, int64_t n, uint32
 }
 
 // Composite sort key of every node (its tile times sub_range plus, for the
-// plane-tiled kernels, its sub_key within the tile; sub_range 1 otherwise),
-// its tile's count, and (plane-tiled kernels) its window descriptor by node
-// index: the FP64 index/fraction split of window_cell, lo_t the window start
-// relative to the origin of the node's tile, in [0, T_t) by construction of
-// the key.
+// plane-tiled kernels, its sub_key within the tile; sub_range 1 otherwise)
+// and (plane-tiled kernels) its window descriptor by node index: the FP64
+// index/fraction split of window_cell, lo_t the window start relative to the
+// origin of the node's tile, in [0, T_t) by construction of the key.  The
+// tile counts come from the sorted keys (k_bin_tile_first / _count), not from
+// atomics here: a node cluster would serialize them on a few counters.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
                                                   uint32_t *__restrict__ keys,
-                                                  NodeDesc<T> *__restrict__ desc_j,
-                                                  uint32_t *__restrict__ tile_count, Geom ge,
+                                                  NodeDesc<T> *__restrict__ desc_j, Geom ge,
                                                   BinGeom bg, uint32_t sub_range)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -94,7 +94,29 @@ __global__ __launch_bounds__(256) void k_bin_keys(const double *__restrict__ x,
         nd.loc = loc;
         desc_j[j] = nd;
     }
-    atomicAdd(&tile_count[tile], 1u);
+}
+
+// First sorted position of every nonempty tile, from the sorted composite keys.
+__global__ __launch_bounds__(256) void k_bin_tile_first(const uint32_t *__restrict__ skeys, int64_t M,
+                                                        uint32_t sub_range, uint32_t *__restrict__ first)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const uint32_t t = skeys[s] / sub_range;
+    if (s == 0 || skeys[s - 1] / sub_range != t) first[t] = (uint32_t)s;
+}
+
+// Node count of every nonempty tile, written at its last sorted position
+// (empty tiles keep the zero they were filled with).
+__global__ __launch_bounds__(256) void k_bin_tile_count(const uint32_t *__restrict__ skeys, int64_t M,
+                                                        uint32_t sub_range,
+                                                        const uint32_t *__restrict__ first,
+                                                        uint32_t *__restrict__ cnt)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const uint32_t t = skeys[s] / sub_range;
+    if (s == M - 1 || skeys[s + 1] / sub_range != t) cnt[t] = (uint32_t)s + 1u - first[t];
 }
 
 // Exclusive scans of the tile counts and of the chunk counts, in three
@@ -233,7 +255,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t *__restri
 }
 
 // vals_in NULL: the values are the item indices (the first pass over node
-// order).  keys_out NULL: the sorted keys are not needed (last pass).
+// order).  keys_out may be NULL when the sorted keys are not needed.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t *__restrict__ keys_in,
                                                            const uint32_t *__restrict__ vals_in,
                                                            int64_t M, int shift,
@@ -465,8 +487,8 @@ static void scan_counts(const uint32_t *cnt, uint32_t *start, uint32_t *chunk_st
 }
 
 // The binning passes: keys (and, for the plane-tiled kernels, descriptors by
-// node), tile scan and chunk list, the radix sort of the keys, and the
-// sorted copies of the per-node data.
+// node), the radix sort of the keys, the tile counts from the sorted keys,
+// tile scan and chunk list, and the sorted copies of the per-node data.
 template <typename T> void Binning::run_build(const Geom &ge, const BinGeom &bg, const double *x,
                                               hipStream_t s)
 {
@@ -474,21 +496,21 @@ template <typename T> void Binning::run_build(const Geom &ge, const BinGeom &bg,
     NodeDesc<T> *dj = planes() ? static_cast<NodeDesc<T> *>(desc2) : nullptr;   // by node
     const uint32_t sr = planes() ? sub_range(bg) : 1u;
     k_fill_u32<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, ntiles, 0u);
-    k_bin_keys<T><<<gm, 256, 0, s>>>(x, keys, dj, tile_count, ge, bg, sr);
-    scan_counts(tile_count, tile_start, chunk_start, n_chunks, scan_part, ntiles, s);
-    k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
-                                                                  chunk_start, chunks, ntiles);
+    k_bin_keys<T><<<gm, 256, 0, s>>>(x, keys, dj, ge, bg, sr);
     // radix passes: pass p reads (kin, vin) and writes (kout, vout); the
     // destinations alternate so that the last pass writes perm
     const uint32_t nblk = (uint32_t)((M + kRsItems - 1) / kRsItems);
     uint32_t *kbuf[2] = {keys, keys2};
     uint32_t *vbuf[2] = {perm, perm2};
+    uint32_t *skeys = nullptr;   // the sorted keys (written by the last pass)
     for (int p = 0; p < sort_passes; ++p) {
         const int dst = (sort_passes - 1 - p) & 1;   // the last pass writes buffer 0 (perm)
         const uint32_t *kin = p == 0 ? keys : kbuf[dst ^ 1];
         const uint32_t *vin = p == 0 ? nullptr : vbuf[dst ^ 1];
-        // the first pass reads keys (buffer 0); it must not also write keys
-        uint32_t *kout = p + 1 < sort_passes ? (p == 0 ? keys2 : kbuf[dst]) : nullptr;
+        // the first pass reads keys (buffer 0); it must not also write keys;
+        // the last one writes its keys into whichever key buffer it does not read
+        uint32_t *kout = p + 1 < sort_passes ? (p == 0 ? keys2 : kbuf[dst]) : (kin == keys ? keys2 : keys);
+        if (p + 1 == sort_passes) skeys = kout;
         uint32_t *vout = vbuf[dst];
         k_rs_hist<<<nblk, kRsThreads, 0, s>>>(kin, M, 8 * p, rs_hist, nblk);
         scan_counts(rs_hist, rs_hist, nullptr, nullptr, scan_part, (int64_t)kRsDigits * nblk, s);
@@ -499,6 +521,13 @@ template <typename T> void Binning::run_build(const Geom &ge, const BinGeom &bg,
             kbuf[dst ^ 1] = keys;
         }
     }
+    // tile counts (tile_start holds each tile's first position until the scan
+    // overwrites it), then the scans and the chunk list
+    k_bin_tile_first<<<gm, 256, 0, s>>>(skeys, M, sr, tile_start);
+    k_bin_tile_count<<<gm, 256, 0, s>>>(skeys, M, sr, tile_start, tile_count);
+    scan_counts(tile_count, tile_start, chunk_start, n_chunks, scan_part, ntiles, s);
+    k_bin_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tile_count, tile_start,
+                                                                  chunk_start, chunks, ntiles);
     if (planes())
         k_bin_gather_desc<T><<<gm, 256, 0, s>>>(perm, dj, static_cast<NodeDesc<T> *>(desc), M);
     else

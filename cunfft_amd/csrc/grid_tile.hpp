@@ -1788,15 +1788,19 @@ __global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__res
 // Wavefronts per workgroup.  A tile small enough for two or more workgroups
 // per CU gets 4 forward wavefronts (node batches); a tile that fills the CU's
 // LDS alone gets 8, so the CU still holds enough wavefronts to cover the LDS
-// latency.  The 3D adjoint runs 4 wavefronts (each owning every fourth
-// plane): with the units held across nodes and the shared weight batches it
-// is bound by instruction issue at 3 workgroups per CU, and the extra
-// wavefronts beat the two dummy planes they add (FP32 1.33 -> 1.30 ms, FP64
-// 3.04 -> 2.81 ms at 256^3 against 2 wavefronts).
+// latency.  The 3D FP32 gather also runs 8: three of its 52 KB workgroups
+// fill a CU's LDS, and 24 wavefronts per CU instead of 12 hide more of each
+// node's LDS round trip (8^3 tiles at 256^3: 0.549 -> 0.488 ms; 16 wavefronts
+// are capped at the same 24 per CU by registers).  The 3D adjoint runs 4
+// wavefronts (each owning every fourth plane): with the units held across
+// nodes and the shared weight batches it is bound by instruction issue at 3
+// workgroups per CU, and the extra wavefronts beat the two dummy planes they
+// add (FP32 1.33 -> 1.30 ms, FP64 3.04 -> 2.81 ms at 256^3 against 2
+// wavefronts).
 template <typename T, int D, int W, int V> struct Waves {
     static constexpr int tile_bytes = TileLayout<T, D, W, 1, V>::CELLS * TileLayout<T, D, W, 1, V>::CB;
     static constexpr bool alone = 2 * (tile_bytes + 1024) > 160 * 1024;
-    static constexpr int gather = alone ? 8 : 4;
+    static constexpr int gather = alone || (D == 3 && sizeof(T) == 4) ? 8 : 4;
     // 2D: a node's window is W rows of W cells, which one wavefront covers
     // with most lanes busy; splitting its rows over wavefronts would leave
     // most lanes idle, so the adjoint runs one wavefront on a small tile

@@ -1689,48 +1689,42 @@ __device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slo
         }
     };
 #pragma unroll
-    for (int q0 = 0; q0 < Q0; ++q0) {
-        // the first chunk of every candidate of this q0 in flight together
-        C v[Q1][QL][SEGC];
-        bool in01[Q1];
-        int rc[Q1];
+    for (int q0 = 0; q0 < Q0; ++q0)
 #pragma unroll
         for (int q1 = 0; q1 < Q1; ++q1) {
             const int E0 = L::ext(L::T0, L::P0, q0), E1 = L::ext(L::T1, L::P1, q1);
-            in01[q1] = r0 < E0 && r1 < E1;
-            rc[q1] = (r0 < E0 ? r0 : 0) * E1 + (r1 < E1 ? r1 : 0);
+            const bool in01 = r0 < E0 && r1 < E1;
             // a halo block pair that covers none of the wavefront's columns
             // (E0 = 1 past the first rows of the tile) is not loaded at all
-            if (!__any(in01[q1])) continue;
+            if (!__any(in01)) continue;
+            const int rc = (r0 < E0 ? r0 : 0) * E1 + (r1 < E1 ? r1 : 0);
+            // the first chunk of the QL candidates of this (q0, q1) in flight together
+            C v[QL][SEGC];
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
                 // a candidate with no nodes reads chunk 0 and adds nothing
                 const C *sp = slots + (int64_t)(nch[q0][q1][qL] > 0 ? c0[q0][q1][qL] : 0) * L::SC;
                 const int EL = L::ext(TL, L::PL, qL);
-                const int o = L::block_off(q0, q1, qL) + rc[q1] * EL;
-                if (EL == TL) load_full(sp, o + seg * SEGC, v[q1][qL]);
-                else load_last(sp, o, v[q1][qL]);
+                const int o = L::block_off(q0, q1, qL) + rc * EL;
+                if (EL == TL) load_full(sp, o + seg * SEGC, v[qL]);
+                else load_last(sp, o, v[qL]);
             }
-        }
-#pragma unroll
-        for (int q1 = 0; q1 < Q1; ++q1) {
-            if (!__any(in01[q1])) continue;
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
                 const int EL = L::ext(TL, L::PL, qL);
-                const bool ok = nch[q0][q1][qL] > 0 && in01[q1];
+                const bool ok = nch[q0][q1][qL] > 0 && in01;
 #pragma unroll
                 for (int i = 0; i < SEGC; ++i)
                     if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && ok) {
-                        fold_add<COMP>(sx[i], cx[i], v[q1][qL][i].x);
-                        fold_add<COMP>(sy[i], cy[i], v[q1][qL][i].y);
+                        fold_add<COMP>(sx[i], cx[i], v[qL][i].x);
+                        fold_add<COMP>(sy[i], cy[i], v[qL][i].y);
                     }
             }
             // further chunks of dense candidate tiles
 #pragma unroll
             for (int qL = 0; qL < QL; ++qL) {
                 const int EL = L::ext(TL, L::PL, qL);
-                const int o = L::block_off(q0, q1, qL) + rc[q1] * EL;
+                const int o = L::block_off(q0, q1, qL) + rc * EL;
                 for (uint32_t c = 1; c < nch[q0][q1][qL]; ++c) {
                     const C *sp = slots + (int64_t)(c0[q0][q1][qL] + c) * L::SC;
                     C w[SEGC];
@@ -1738,14 +1732,13 @@ __device__ __forceinline__ void tile_fold_cols(const cplx_t<T> *__restrict__ slo
                     else load_last(sp, o, w);
 #pragma unroll
                     for (int i = 0; i < SEGC; ++i)
-                        if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && in01[q1]) {
+                        if ((EL == TL || (i < NLAST && seg * SEGC + i < EL)) && in01) {
                             fold_add<COMP>(sx[i], cx[i], w[i].x);
                             fold_add<COMP>(sy[i], cy[i], w[i].y);
                         }
                 }
             }
         }
-    }
     const uint32_t g1 = D == 3 ? (uint32_t)ge.n[1] : 1u, gL = (uint32_t)ge.n[D - 1];
     C *dst = g + (size_t)((tk[0] * L::T0 + r0) * g1 + tk[1] * L::T1 + r1) * gL + tk[2] * TL + seg * SEGC;
 #pragma unroll

@@ -401,7 +401,7 @@ static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 //   keys, keys2, perm, perm2    M u32 each
 //   rs_hist                     256 x radix blocks u32
 //   scan_part                   2 x the blocks of the larger scan
-//   tile_count/_start, chunk_start  ntiles u32 each; n_chunks 1
+//   tile_count/_start, chunk_start  ntiles u32 each; n_chunks 2 (count, work counter)
 //   chunks                      max_chunks
 //   plane-tiled: desc, desc2    M descriptors each; slots, fs (own scratch)
 //   node-lane:   xs             M d doubles
@@ -424,7 +424,7 @@ static size_t bin_layout(Binning &b, int64_t M, char *base)
     uint32_t *tile_count = (uint32_t *)take(4 * (size_t)b.ntiles);
     uint32_t *tile_start = (uint32_t *)take(4 * (size_t)b.ntiles);
     uint32_t *chunk_start = (uint32_t *)take(4 * (size_t)b.ntiles);
-    uint32_t *n_chunks = (uint32_t *)take(4);
+    uint32_t *n_chunks = (uint32_t *)take(8);   // chunk count, then the spread's work counter
     Chunk *chunks = (Chunk *)take(sizeof(Chunk) * (size_t)max_chunks);
     void *desc = nullptr, *desc2 = nullptr, *slots = nullptr, *fs = nullptr;
     double *xs = nullptr;

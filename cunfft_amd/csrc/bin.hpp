@@ -123,7 +123,7 @@ struct Binning {
     uint32_t *tile_count = nullptr; // ntiles
     uint32_t *tile_start = nullptr; // ntiles
     uint32_t *chunk_start = nullptr;// ntiles
-    uint32_t *n_chunks = nullptr;   // 1
+    uint32_t *n_chunks = nullptr;   // 2: the chunk count, then the persistent spread's work counter
     Chunk *chunks = nullptr;        // max_chunks
     double *xs = nullptr;           // M*d: nodes in sorted order (node-lane kernels)
     void *desc = nullptr;           // M: NodeDesc<T> in sorted order (plane-tiled kernels)

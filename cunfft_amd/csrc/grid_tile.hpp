@@ -26,6 +26,8 @@
 // slot blocks of the chunks whose padded tiles cover it: no atomics, no zero
 // fill, and a result that does not depend on scheduling.
 #pragma once
+#include <atomic>
+
 #include "bin.hpp"
 
 namespace cunfft {
@@ -1220,6 +1222,21 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
     tile_to_slot<L, COMP, NT>(tile, slot);
 }
 
+// Work-queue state of the persistent held-unit spread (k_tile_spread_queue):
+// workgroups take chunks from a counter, and each prefetches the first nodes
+// of its next chunk while the current tile goes to its slot.  Thread 0 takes
+// the next ticket when a chunk starts and publishes it in LDS only after the
+// node loop, so the atomic's round trip hides behind the chunk's work.
+template <typename T> struct SpreadQueue {
+    uint32_t *counter;      // work counter, zero at launch
+    const Chunk *chunks;
+    uint32_t n;             // chunk count
+    uint32_t *s_next;       // LDS: the next chunk, published by thread 0
+    uint32_t ticket;        // thread 0: its next chunk
+    NodeDesc<T> nd;         // lanes < NB: the chunk's first nodes, prefetched
+    cplx_t<T> fv;
+};
+
 // Adjoint with the tile units held across nodes (d = 3, one plane per lane
 // group).  The nodes of a tile come sorted by row, column unit and plane of
 // their window start, so consecutive nodes often cover the same rows and
@@ -1231,10 +1248,11 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
 // of one.  The plane weights come grouped by residue (zero past the window),
 // so a node whose wavefront planes run past its window adds zero there.  Order
 // of the sums per cell: the nodes in sorted order, as in the direct update.
-template <typename T, int D, int W, int NW, int V>
+template <typename T, int D, int W, int NW, int V, bool Q = false>
 __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__ desc,
                                                  const cplx_t<T> *__restrict__ fs, const Chunk ch,
-                                                 cplx_t<T> *__restrict__ slot, const WinConst<T> &wc)
+                                                 cplx_t<T> *__restrict__ slot, const WinConst<T> &wc,
+                                                 SpreadQueue<T> *q = nullptr)
 {
     using L = TileLayout<T, D, W, NW, V>;
     static_assert(D == 3 && L::G == 1, "units held across nodes: one plane per lane group");
@@ -1255,7 +1273,11 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
 
     NodeDesc<T> nd{};
     C fv = cmake<T>(0, 0);
-    if (lane < NB && (uint32_t)lane < ch.count) {
+    if constexpr (Q) {
+        nd = q->nd;   // prefetched during the previous chunk's slot write
+        fv = q->fv;
+        if (tid == 0) q->ticket = atomicAdd(q->counter, 1u);
+    } else if (lane < NB && (uint32_t)lane < ch.count) {
         nd = desc[ch.start + lane];
         fv = fs[ch.start + lane];
     }
@@ -1393,7 +1415,21 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
             for (int i = 0; i < OWN; ++i) *reinterpret_cast<VT *>(tb + ad[ii][i]) = v[ii][i];
         }
     }
+    if constexpr (Q) {
+        if (tid == 0) *q->s_next = q->ticket;
+    }
     __syncthreads();
+    if constexpr (Q) {
+        // the next chunk's first nodes, in flight during the slot write
+        const uint32_t cn = *q->s_next;
+        if (cn < q->n) {
+            const Chunk nc = q->chunks[cn];
+            if (lane < NB && (uint32_t)lane < nc.count) {
+                q->nd = desc[nc.start + lane];
+                q->fv = fs[nc.start + lane];
+            }
+        }
+    }
     tile_to_slot<L, false, NT>(tile, slot);
 }
 
@@ -1411,6 +1447,48 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
     else
         tile_spread<T, D, W, NW, V, COMP>(desc, fs, chunks[blockIdx.x],
                                           slots + (int64_t)blockIdx.x * L::SC, wc);
+}
+
+// Persistent form of the 3D held-unit spread: a grid of resident workgroups
+// takes chunks from the binning's work counter (n_chunks[1], cleared before
+// the launch) until it runs past the chunk count, so a chunk's first node
+// loads overlap the previous chunk's slot write and no chunk waits for a
+// workgroup launch.  Every workgroup takes one ticket past the last chunk and
+// exits.
+template <typename T, int D, int W, int NW, int V>
+__global__ __launch_bounds__(NW * 64) void k_tile_spread_queue(
+    const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
+    const Chunk *__restrict__ chunks, uint32_t *__restrict__ n_chunks,
+    cplx_t<T> *__restrict__ slots, WinConst<T> wc)
+{
+    using L = TileLayout<T, D, W, NW, V>;
+    __shared__ uint32_t s_next;
+    const int tid = threadIdx.x, lane = tid & 63;
+    SpreadQueue<T> q;
+    q.counter = n_chunks + 1;
+    q.chunks = chunks;
+    q.n = n_chunks[0];
+    q.s_next = &s_next;
+    q.ticket = 0;
+    q.nd = NodeDesc<T>{};
+    q.fv = cmake<T>(0, 0);
+    if (tid == 0) s_next = atomicAdd(q.counter, 1u);
+    __syncthreads();
+    uint32_t c = s_next;
+    if (c < q.n) {
+        const Chunk ch = chunks[c];
+        constexpr int TILE_UNITS = L::CELLS / L::CPL + L::DUMMY_UNITS;
+        constexpr int NB = batch_nodes<T, D, W, NW, 32>(TILE_UNITS * 16, 1, 0);
+        if (lane < NB && (uint32_t)lane < ch.count) {
+            q.nd = desc[ch.start + lane];
+            q.fv = fs[ch.start + lane];
+        }
+    }
+    while (c < q.n) {
+        tile_spread_runs<T, D, W, NW, V, true>(desc, fs, chunks[c], slots + (int64_t)c * L::SC, wc, &q);
+        c = s_next;        // published before the chunk's last barrier
+        __syncthreads();   // the slot write's LDS reads are done before the next chunk clears the tile
+    }
 }
 
 template <typename T, int D, int W, int NW, int V>
@@ -1932,6 +2010,29 @@ template <typename T> struct GradOp {
     }
 };
 
+// (static: each precision's translation unit has its own copy)
+static __global__ void k_clear_u32(uint32_t *p)
+{
+    if (threadIdx.x == 0) p[0] = 0u;
+}
+
+// Workgroups of kernel k (threads per workgroup, no dynamic LDS) that the
+// current device holds at once: its CUs times the occupancy per CU (asked
+// once per device and kernel).
+template <typename K> static int64_t resident_blocks(K k, int threads)
+{
+    static std::atomic<int64_t> known[64] = {};
+    int dev = 0;
+    CUNFFT_HIP(hipGetDevice(&dev));
+    if (dev >= 0 && dev < 64 && known[dev].load() > 0) return known[dev].load();
+    int cus = 0, per = 0;
+    CUNFFT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CUNFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, 0));
+    const int64_t r = (int64_t)std::max(1, cus) * std::max(1, per);
+    if (dev >= 0 && dev < 64) known[dev].store(r);
+    return r;
+}
+
 template <typename T> struct SpreadOp {
     const Geom &ge;
     const Binning &b;
@@ -1963,9 +2064,20 @@ template <typename T> struct SpreadOp {
                     return true;
                 }
             }
-            k_tile_spread<T, D, W, NWS, V, false><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
-                static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
-                win_const<T>(ge));
+            if constexpr (D == 3 && L::G == 1) {
+                // persistent held-unit spread: clear the work counter (a kernel, not a
+                // memset, so the launch sequence can be captured in a graph), then one
+                // resident grid
+                auto kq = k_tile_spread_queue<T, D, W, NWS, V>;
+                k_clear_u32<<<1, 64, 0, s>>>(b.n_chunks + 1);
+                const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(kq, NWS * 64));
+                kq<<<grid, NWS * 64, 0, s>>>(static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks,
+                                             b.n_chunks, slots, win_const<T>(ge));
+            } else {
+                k_tile_spread<T, D, W, NWS, V, false><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
+                    static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
+                    win_const<T>(ge));
+            }
             if (mid) CUNFFT_HIP(hipEventRecord(mid, s));
             k_tile_fold<T, D, W, NWS, V, false><<<fold_grid, 256, 0, s>>>(
                 slots, b.tile_count, b.chunk_start, g, ge, bin_geom(b));

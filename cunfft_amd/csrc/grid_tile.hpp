@@ -1224,12 +1224,18 @@ __device__ __forceinline__ void tile_spread(const NodeDesc<T> *__restrict__ desc
 
 // Work-queue state of the persistent held-unit spread (k_tile_spread_queue):
 // workgroups take chunks from a counter, and each prefetches the first nodes
-// of its next chunk while the current tile goes to its slot.  Thread 0 takes
+// of its next chunk (descriptors and coefficients) while the current tile
+// goes to its slot.  FP32 reads the coefficients through the permutation,
+// which saves the sorted copy of f; FP64, at two workgroups per CU, hides
+// the dependent 16-byte loads less well and keeps the copy.  Thread 0 takes
 // the next ticket when a chunk starts and publishes it in LDS only after the
 // node loop, so the atomic's round trip hides behind the chunk's work.
 template <typename T> struct SpreadQueue {
     uint32_t *counter;      // work counter, zero at launch
     const Chunk *chunks;
+    const uint32_t *perm;   // node of each sorted position: the coefficients are read
+    const cplx_t<T> *f;     // through it (FP32), or from the sorted copy fs (perm NULL)
+    const cplx_t<T> *fs;
     uint32_t n;             // chunk count
     uint32_t *s_next;       // LDS: the next chunk, published by thread 0
     uint32_t ticket;        // thread 0: its next chunk
@@ -1317,7 +1323,8 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
             const uint32_t s = s0 + NB + lane;
             if (lane < NB && s < ch.count) {
                 nd = desc[ch.start + s];
-                fv = fs[ch.start + s];
+                if constexpr (Q) fv = q->perm ? q->f[q->perm[ch.start + s]] : q->fs[ch.start + s];
+                else fv = fs[ch.start + s];
             }
         }
         // lane n's view of node n for this wavefront: key (row/column base and
@@ -1426,7 +1433,7 @@ __device__ __forceinline__ void tile_spread_runs(const NodeDesc<T> *__restrict__
             const Chunk nc = q->chunks[cn];
             if (lane < NB && (uint32_t)lane < nc.count) {
                 q->nd = desc[nc.start + lane];
-                q->fv = fs[nc.start + lane];
+                q->fv = q->perm ? q->f[q->perm[nc.start + lane]] : q->fs[nc.start + lane];
             }
         }
     }
@@ -1457,9 +1464,9 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread(
 // exits.
 template <typename T, int D, int W, int NW, int V>
 __global__ __launch_bounds__(NW * 64) void k_tile_spread_queue(
-    const NodeDesc<T> *__restrict__ desc, const cplx_t<T> *__restrict__ fs,
-    const Chunk *__restrict__ chunks, uint32_t *__restrict__ n_chunks,
-    cplx_t<T> *__restrict__ slots, WinConst<T> wc)
+    const NodeDesc<T> *__restrict__ desc, const uint32_t *__restrict__ perm,
+    const cplx_t<T> *__restrict__ f, const cplx_t<T> *__restrict__ fs, const Chunk *__restrict__ chunks,
+    uint32_t *__restrict__ n_chunks, cplx_t<T> *__restrict__ slots, WinConst<T> wc)
 {
     using L = TileLayout<T, D, W, NW, V>;
     __shared__ uint32_t s_next;
@@ -1467,6 +1474,9 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread_queue(
     SpreadQueue<T> q;
     q.counter = n_chunks + 1;
     q.chunks = chunks;
+    q.perm = perm;
+    q.f = f;
+    q.fs = fs;
     q.n = n_chunks[0];
     q.s_next = &s_next;
     q.ticket = 0;
@@ -1481,11 +1491,12 @@ __global__ __launch_bounds__(NW * 64) void k_tile_spread_queue(
         constexpr int NB = batch_nodes<T, D, W, NW, 32>(TILE_UNITS * 16, 1, 0);
         if (lane < NB && (uint32_t)lane < ch.count) {
             q.nd = desc[ch.start + lane];
-            q.fv = fs[ch.start + lane];
+            q.fv = perm ? f[perm[ch.start + lane]] : fs[ch.start + lane];
         }
     }
     while (c < q.n) {
-        tile_spread_runs<T, D, W, NW, V, true>(desc, fs, chunks[c], slots + (int64_t)c * L::SC, wc, &q);
+        tile_spread_runs<T, D, W, NW, V, true>(desc, nullptr, chunks[c], slots + (int64_t)c * L::SC, wc,
+                                               &q);
         c = s_next;        // published before the chunk's last barrier
         __syncthreads();   // the slot write's LDS reads are done before the next chunk clears the tile
     }
@@ -2046,8 +2057,12 @@ template <typename T> struct SpreadOp {
             constexpr int NWS = Waves<T, D, W, V>::spread;
             cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
             cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
-            k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
             using L = TileLayout<T, D, W, NWS, V>;
+            // the FP32 3D held-unit spread reads f through the permutation itself
+            constexpr bool QUEUE = D == 3 && L::G == 1;
+            constexpr bool PERM_F = QUEUE && sizeof(T) == 4;
+            if (!PERM_F || b.compensated)
+                k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
             constexpr int CORE = L::T0 * L::T1 * L::TL;
             constexpr int SLICES = CORE >= 512 ? CORE / 512 : 1;
             const dim3 fold_grid = kFoldColumns<T, D>
@@ -2064,15 +2079,16 @@ template <typename T> struct SpreadOp {
                     return true;
                 }
             }
-            if constexpr (D == 3 && L::G == 1) {
+            if constexpr (QUEUE) {
                 // persistent held-unit spread: clear the work counter (a kernel, not a
                 // memset, so the launch sequence can be captured in a graph), then one
                 // resident grid
                 auto kq = k_tile_spread_queue<T, D, W, NWS, V>;
                 k_clear_u32<<<1, 64, 0, s>>>(b.n_chunks + 1);
                 const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(kq, NWS * 64));
-                kq<<<grid, NWS * 64, 0, s>>>(static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks,
-                                             b.n_chunks, slots, win_const<T>(ge));
+                kq<<<grid, NWS * 64, 0, s>>>(static_cast<const NodeDesc<T> *>(b.desc),
+                                             PERM_F ? b.perm : nullptr, f, fs, b.chunks, b.n_chunks,
+                                             slots, win_const<T>(ge));
             } else {
                 k_tile_spread<T, D, W, NWS, V, false><<<(unsigned)b.max_chunks, NWS * 64, 0, s>>>(
                     static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,

@@ -1,0 +1,331 @@
+// Adjoint tile accumulation on the matrix cores (FP32, d = 3, 8^3 tiles).
+//
+// Reference behaviour: the adjoint gridding loop of the reference's
+// spreading kernels (src/cunfft_kernel.cu, the adjoint convolution over the
+// 2m+2 window points per axis) sums f_j phi(x_j - cell) into every cell of
+// each node's window.  Here the sum over the nodes of one chunk is a matrix
+// product.  With the padded tile's cells as (plane a, row b, column c) and the
+// window weights separable,
+//
+//   tile[a][b][c] = sum_j  wa_j[a] * (f_j wb_j[b] wc_j[c])
+//                 = sum_j  A[a][j] * B[j][(b, c, re/im)],
+//
+// A (planes x nodes) holding the plane weights and B (nodes x columns) the
+// products of coefficient, row and column weight, real and imaginary part in
+// adjacent columns.  v_mfma_f32_16x16x4f32 multiplies 16 planes by 4 nodes by
+// 16 columns, accumulating in FP32 (FP32 products, as the VALU form).
+//
+// * Planes.  The padded tile has P = 8 + W - 1 planes (17 at W = 10) but one
+//   instruction covers 16.  A node's window spans W planes from its start
+//   lo_0 in [0, 8), so the nodes are split into two sets: lo_0 <= 16 - W
+//   (window inside planes [0, 16)) and the rest (inside [P - 16, P)).  Each
+//   set has its own accumulators whose rows are its 16 planes, and each node
+//   goes through the matrix cores once.  The sets are summed at the end.
+// * Columns.  Each wavefront owns a band of rows b.  Per row it holds two
+//   blocks of 16 columns (columns c in [0, 8) and [8, 16), re/im), and the
+//   columns c >= 16 of all rows (P - 16 of them) are packed into extra
+//   blocks spread over the wavefronts.
+// * Per batch of 64 nodes the weights go to LDS once, in padded-tile
+//   coordinates (zero outside the window): plane weights per set row, row
+//   weights times the coefficient (re and im rows), column weights.  The
+//   nodes of a set are padded to a multiple of 4 with zero rows.  Wavefront t
+//   < 3 evaluates axis t for the batch's nodes (lane n: node n), wavefront 3
+//   clears the padding rows.
+// * The accumulators go to an LDS image of the chunk's slot (the fold's
+//   destination-blocked order), which the workgroup then copies out with
+//   16-byte stores: whole cache lines, where storing the accumulators
+//   directly (4-byte lanes, 64-byte runs) took more than the matrix work.
+//
+// The sums per cell run over the nodes of the chunk in a fixed order (sorted
+// order within a set, in the instruction's fixed reduction order), so the
+// result does not depend on scheduling.
+#pragma once
+
+namespace cunfft {
+
+typedef float mf4_t __attribute__((ext_vector_type(4)));
+
+template <int W> struct MfmaSpread {
+    static constexpr int T = 8;                         // tile extent per axis
+    static constexpr int P = T + W - 1;                 // padded extent per axis
+    static constexpr int OFF1 = P > 16 ? P - 16 : 0;    // first plane of the second set
+    static constexpr bool SETS2 = OFF1 > 0;
+    static constexpr int LO_MAX0 = 16 - W;              // set 0: window starts lo_0 <= LO_MAX0
+    static constexpr int NW = 4;                        // wavefronts per workgroup
+    static constexpr int RW = (P + NW - 1) / NW;        // rows per wavefront (at most)
+    static constexpr int NLO = P > 16 ? P - 16 : 0;     // columns past the two 8-column halves
+    static constexpr int NLOB = (P * NLO * 2 + 15) / 16;   // their 16-column blocks
+    static constexpr int LB = (NLOB + NW - 1) / NW;     // of those, per wavefront (at most)
+    static constexpr int NB = 64;                       // nodes per batch
+    static constexpr int NPOS = NB + 8;                 // batch rows: each set padded to 4 nodes
+    static constexpr int RS = 20;                       // row / column weight table stride
+    static_assert(W >= 2 && 2 * W + T <= 32, "every window start fits one of the two plane sets");
+    static_assert(P <= RS, "a padded row fits the weight table stride");
+};
+
+template <int W> constexpr bool kMfmaSpread = W >= 4 && W <= 12;
+
+// Float offsets in a slot of the 8^3 padded tile (the destination-blocked
+// order of TileLayout::block_off) of part ri of the cells (a, b, c) a lane
+// group g stores: planes a = 4g + i (i < 4, a < 16) at base + i * step, and
+// the planes a = 16 + i past them at base16 + i * step.
+struct MfmaSlotCol {
+    int base, base16, step;
+};
+template <typename L> __device__ __forceinline__ MfmaSlotCol mfma_slot_col(int g, int b, int c, int ri)
+{
+    const int q1 = b >> 3, qL = c >> 3;
+    const int E1 = L::ext(L::T1, L::P1, q1), EL = L::ext(L::TL, L::PL, qL);
+    const int in = (b & 7) * EL + (c & 7);   // row and column within the block
+    MfmaSlotCol o;
+    o.base = 2 * (L::block_off(g >> 1, q1, qL) + 4 * (g & 1) * E1 * EL + in) + ri;
+    o.base16 = 2 * (L::block_off(2, q1, qL) + in) + ri;
+    o.step = 2 * E1 * EL;
+    return o;
+}
+
+// Persistent: a grid of resident workgroups takes chunks from the work
+// counter n_chunks[1] (zero at launch) until it runs past the chunk count
+// n_chunks[0]; every workgroup takes one ticket past the last chunk and
+// exits.  The next batch's nodes (or the next chunk's first batch) load
+// while the current one runs through the matrix cores, and the operands of
+// the next 4-node step load while the current step's instructions issue.
+// WIN: the window (inline, no calls: the accumulators stay in registers
+// through the weight evaluation).
+template <int W, int WIN>
+__global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<float> *__restrict__ desc,
+                                                          const cplx_t<float> *__restrict__ fs,
+                                                          const Chunk *__restrict__ chunks,
+                                                          uint32_t *__restrict__ n_chunks,
+                                                          cplx_t<float> *__restrict__ slots,
+                                                          WinConst<float> wc)
+{
+    using S = MfmaSpread<W>;
+    using L = TileLayout<float, 3, W, 4, 0>;   // slot layout (shared with the fold)
+    static_assert(L::T0 == S::T && L::T1 == S::T && L::TL == S::T && L::P0 == S::P, "8^3 tiles");
+    constexpr int LBN = S::LB > 0 ? S::LB : 1;
+    __shared__ __align__(16) float tA[S::NPOS][16];
+    __shared__ __align__(16) float tB1[S::NPOS][2][S::RS];
+    __shared__ __align__(16) float tB2[S::NPOS][S::RS];
+    __shared__ __align__(16) float stg[2 * L::SC];   // the slot image
+    __shared__ uint32_t s_next;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // operand lanes: A[m = lane % 16][k = lane / 16], B[k = lane / 16][n = lane % 16],
+    // accumulator register i: D[4 (lane / 16) + i][lane % 16]
+    const int g = lane >> 4, nn = lane & 15, ri = nn & 1, cc = nn >> 1;
+    const int rb = wave * S::P / S::NW;
+    const int nrows = (wave + 1) * S::P / S::NW - rb;
+    // the lane's column of each extra block: row lb_b, column lb_c (>= 16)
+    int lb_b[LBN], lb_c[LBN];
+    bool lb_ok[LBN];
+#pragma unroll
+    for (int j = 0; j < LBN; ++j) {
+        const int blk = wave + S::NW * j;
+        const int e = (16 * blk + nn) >> 1;
+        const int b = e / (S::NLO > 0 ? S::NLO : 1);
+        lb_ok[j] = S::LB > 0 && blk < S::NLOB && b < S::P;
+        lb_b[j] = lb_ok[j] ? b : 0;
+        lb_c[j] = lb_ok[j] ? 16 + e % (S::NLO > 0 ? S::NLO : 1) : 0;
+    }
+    uint32_t *counter = n_chunks + 1;
+    const uint32_t n = n_chunks[0];
+    if (tid == 0) s_next = atomicAdd(counter, 1u);
+    __syncthreads();
+    uint32_t c = s_next;
+    // lane n < 64: node n of the next batch to run
+    NodeDesc<float> nd{};
+    cplx_t<float> fv = cmake<float>(0.f, 0.f);
+    Chunk ch{0u, 0u, 0u, 0u};
+    if (c < n) {
+        ch = chunks[c];
+        if ((uint32_t)lane < ch.count) {
+            nd = desc[ch.start + lane];
+            fv = fs[ch.start + lane];
+        }
+    }
+
+    // operands of one 4-node step
+    struct Ops {
+        float a, w2lo, w2hi, w1[S::RW], x1[LBN], x2[LBN];
+    };
+    auto load_ops = [&](int ks) {
+        Ops o;
+        const int p = 4 * ks + g;
+        o.a = tA[p][nn];
+        o.w2lo = tB2[p][cc];
+        o.w2hi = tB2[p][8 + cc];
+#pragma unroll
+        for (int r = 0; r < S::RW; ++r) o.w1[r] = r < nrows ? tB1[p][ri][rb + r] : 0.f;
+#pragma unroll
+        for (int j = 0; j < S::LB; ++j) {
+            o.x1[j] = tB1[p][ri][lb_b[j]];
+            o.x2[j] = tB2[p][lb_c[j]];
+        }
+        return o;
+    };
+    const mf4_t z4 = {0.f, 0.f, 0.f, 0.f};
+    mf4_t acc0[S::RW][2], acc1[S::RW][2], accL0[LBN], accL1[LBN];
+    auto mma = [&](const Ops &o, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+#pragma unroll
+        for (int r = 0; r < S::RW; ++r) {
+            if (r >= nrows) break;   // wavefront-uniform
+            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2lo, acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2hi, acc[r][1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < S::LB; ++j) {
+            if (wave + S::NW * j >= S::NLOB) break;   // wavefront-uniform
+            const float bv = lb_ok[j] ? o.x1[j] * o.x2[j] : 0.f;
+            accL[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, bv, accL[j], 0, 0, 0);
+        }
+    };
+
+    // the steps [k0, k1) of one set, the next step's operands in flight
+    auto run_set = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+        if (k0 >= k1) return;
+        Ops cur = load_ops(k0);
+        for (int ks = k0; ks < k1; ++ks) {
+            const Ops nxt = load_ops(ks + 1 < k1 ? ks + 1 : ks);
+            mma(cur, acc, accL);
+            cur = nxt;
+        }
+    };
+
+    uint32_t ticket = 0;
+    while (c < n) {
+#pragma unroll
+        for (int r = 0; r < S::RW; ++r) acc0[r][0] = acc0[r][1] = acc1[r][0] = acc1[r][1] = z4;
+#pragma unroll
+        for (int j = 0; j < LBN; ++j) accL0[j] = accL1[j] = z4;
+        if (tid == 0) ticket = atomicAdd(counter, 1u);
+        for (uint32_t s0 = 0; s0 < ch.count; s0 += S::NB) {
+            const int nb = __builtin_amdgcn_readfirstlane(
+                ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB);
+            if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch's tables
+            const bool valid = lane < nb;
+            const uint32_t loc = nd.loc;
+            const int lo0 = loc & 255, lo1 = (loc >> 8) & 255, loL = (loc >> 16) & 255;
+            const bool s1 = S::SETS2 && valid && lo0 > S::LO_MAX0;
+            const uint64_t m1 = __ballot(s1), m0 = __ballot(valid && !s1);
+            const int n0 = __popcll(m0), n1 = __popcll(m1);
+            const int p0pad = (n0 + 3) & ~3, npos = p0pad + ((n1 + 3) & ~3);
+            const uint64_t mine = s1 ? m1 : m0;
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+            const int pos = s1 ? p0pad + rank : rank;
+            if (wave < 3) {
+                if (valid) {
+                    const int t = wave;   // axis
+                    const float ft = t == 0 ? nd.f[0] : (t == 1 ? nd.f[1] : nd.f[2]);
+                    const float kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
+                    const float nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
+                    const float shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
+                    // cell x of the table row has tap l = x - lo (weight zero outside [0, W))
+                    const int lo = t == 0 ? lo0 - (s1 ? S::OFF1 : 0) : (t == 1 ? lo1 : loL);
+                    const int nx = t == 0 ? 16 : S::RS;
+                    float *row = t == 0 ? &tA[pos][0] : (t == 1 ? &tB1[pos][0][0] : &tB2[pos][0]);
+                    const float sc0 = t == 1 ? fv.x : 1.f, sc1 = fv.y;
+#pragma unroll 1
+                    for (int x0 = 0; x0 < nx; x0 += 4) {
+                        float v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int l = x0 + u - lo;
+                            const float sv = ft + (float)(wc.m - l);
+                            float w;
+                            if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<float>(sv, (float)wc.m, shpt);
+                            else if constexpr (WIN == WIN_SINC) w = sinc_phi<float>(sv, wc.m, shpt);
+                            else w = nrt * wexp<float>(-kbt * sv * sv);
+                            v[u] = l >= 0 && l < W ? w : 0.f;
+                        }
+                        *reinterpret_cast<float4 *>(row + x0) =
+                            make_float4(v[0] * sc0, v[1] * sc0, v[2] * sc0, v[3] * sc0);
+                        if (t == 1)
+                            *reinterpret_cast<float4 *>(row + S::RS + x0) =
+                                make_float4(v[0] * sc1, v[1] * sc1, v[2] * sc1, v[3] * sc1);
+                    }
+                }
+            } else {
+                // zero rows for the padding positions [n0, p0pad) and [p0pad + n1, npos)
+                constexpr int U = 4 + 2 * S::RS / 4 + S::RS / 4;   // 16-byte units per position
+                const int pad0 = p0pad - n0, npad = npos - n0 - n1;
+                for (int i = lane; i < npad * U; i += 64) {
+                    const int pi = i / U, u = i - pi * U;
+                    const int pp = pi < pad0 ? n0 + pi : p0pad + n1 + (pi - pad0);
+                    float4 *d = u < 4 ? reinterpret_cast<float4 *>(&tA[pp][0]) + u
+                              : u < 4 + 2 * S::RS / 4
+                                  ? reinterpret_cast<float4 *>(&tB1[pp][0][0]) + (u - 4)
+                                  : reinterpret_cast<float4 *>(&tB2[pp][0]) + (u - 4 - 2 * S::RS / 4);
+                    *d = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
+            __syncthreads();
+            // the next batch of this chunk loads during this one's steps
+            if (s0 + S::NB < ch.count) {
+                const uint32_t sn = s0 + S::NB + (uint32_t)lane;
+                if (sn < ch.count) {
+                    nd = desc[ch.start + sn];
+                    fv = fs[ch.start + sn];
+                }
+            }
+            run_set(0, p0pad >> 2, acc0, accL0);
+            if constexpr (S::SETS2) run_set(p0pad >> 2, npos >> 2, acc1, accL1);
+        }
+        // the sum of the two sets per plane into the slot image; lane group g holds
+        // planes 4g + i of set 0 and 4g + i + OFF1 of set 1
+        auto put = [&](const mf4_t &v0, const mf4_t &v1, int b, int cl, bool ok) {
+            float up[4];   // set 1 at plane 4g + i
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (!S::SETS2) {
+                    up[i] = 0.f;
+                } else if (i >= S::OFF1) {
+                    up[i] = v1[i - S::OFF1];
+                } else {
+                    const float t = __shfl(v1[4 - S::OFF1 + i], lane - 16);
+                    up[i] = g > 0 ? t : 0.f;
+                }
+            }
+            if (!ok) return;
+            const MfmaSlotCol o = mfma_slot_col<L>(g, b, cl, ri);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * g + i < S::P) stg[o.base + i * o.step] = v0[i] + up[i];
+            if (S::SETS2 && g == 3) {
+#pragma unroll
+                for (int i = 0; i < S::OFF1; ++i) stg[o.base16 + i * o.step] = v1[4 - S::OFF1 + i];
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < S::RW; ++r) {
+            if (r >= nrows) break;
+            put(acc0[r][0], acc1[r][0], rb + r, cc, cc < S::P);
+            put(acc0[r][1], acc1[r][1], rb + r, 8 + cc, 8 + cc < S::P);
+        }
+#pragma unroll
+        for (int j = 0; j < S::LB; ++j) {
+            if (wave + S::NW * j >= S::NLOB) break;
+            put(accL0[j], accL1[j], lb_b[j], lb_c[j], lb_ok[j]);
+        }
+        if (tid == 0) s_next = ticket;
+        __syncthreads();   // the image is whole and the ticket visible
+        const uint32_t cur_c = c;
+        c = s_next;
+        // the next chunk's first batch loads during the copy-out
+        if (c < n) {
+            ch = chunks[c];
+            if ((uint32_t)lane < ch.count) {
+                nd = desc[ch.start + lane];
+                fv = fs[ch.start + lane];
+            }
+        }
+        // the next chunk writes the image only after its first batch's barrier,
+        // which every wavefront reaches after its copy-out reads
+        float4 *dst = reinterpret_cast<float4 *>(slots + (int64_t)cur_c * L::SC);
+        const float4 *src = reinterpret_cast<const float4 *>(stg);
+        for (int i = tid; i < L::SC / 2; i += 256) stream_store(dst + i, src[i]);
+    }
+}
+
+}  // namespace cunfft

@@ -1866,6 +1866,12 @@ __global__ __launch_bounds__(256) void k_tile_fold_batch(const GridJob<T> *__res
                                          bm.blk / SLICES, (int)(bm.blk % SLICES));
 }
 
+}  // namespace cunfft
+
+#include "grid_mfma.hpp"
+
+namespace cunfft {
+
 // ------------------------------------------------------------ dispatch --
 // Wavefronts per workgroup.  A tile small enough for two or more workgroups
 // per CU gets 4 forward wavefronts (node batches); a tile that fills the CU's
@@ -2058,9 +2064,10 @@ template <typename T> struct SpreadOp {
             cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
             cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
             using L = TileLayout<T, D, W, NWS, V>;
-            // the FP32 3D held-unit spread reads f through the permutation itself
+            // the FP32 3D spreads read f through the permutation themselves
+            constexpr bool MFMA = sizeof(T) == 4 && D == 3 && V == 0 && kMfmaSpread<W>;
             constexpr bool QUEUE = D == 3 && L::G == 1;
-            constexpr bool PERM_F = QUEUE && sizeof(T) == 4;
+            constexpr bool PERM_F = QUEUE && !MFMA && sizeof(T) == 4;
             if (!PERM_F || b.compensated)
                 k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
             constexpr int CORE = L::T0 * L::T1 * L::TL;
@@ -2079,7 +2086,18 @@ template <typename T> struct SpreadOp {
                     return true;
                 }
             }
-            if constexpr (QUEUE) {
+            if constexpr (MFMA) {
+                // persistent: clear the work counter (a kernel, not a memset), then one
+                // resident grid
+                auto km = ge.win == WIN_KAISER_BESSEL ? k_tile_spread_mfma<W, WIN_KAISER_BESSEL>
+                          : ge.win == WIN_SINC        ? k_tile_spread_mfma<W, WIN_SINC>
+                                                      : k_tile_spread_mfma<W, WIN_GAUSSIAN>;
+                k_clear_u32<<<1, 64, 0, s>>>(b.n_chunks + 1);
+                const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(km, 256));
+                km<<<grid, 256, 0, s>>>(reinterpret_cast<const NodeDesc<float> *>(b.desc),
+                                        reinterpret_cast<const cplx_t<float> *>(fs), b.chunks, b.n_chunks,
+                                        reinterpret_cast<cplx_t<float> *>(slots), win_const<float>(ge));
+            } else if constexpr (QUEUE) {
                 // persistent held-unit spread: clear the work counter (a kernel, not a
                 // memset, so the launch sequence can be captured in a graph), then one
                 // resident grid

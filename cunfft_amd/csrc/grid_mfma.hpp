@@ -26,8 +26,9 @@
 //   columns c >= 16 of all rows (P - 16 of them) are packed into extra
 //   blocks spread over the wavefronts.
 // * Per batch of 64 nodes the weights go to LDS once, in padded-tile
-//   coordinates (zero outside the window): plane weights per set row, row
-//   weights times the coefficient (re and im rows), column weights.  The
+//   coordinates (zero outside the window), one table row per node: plane
+//   weights per set row, row weights times the coefficient (re and im), column
+//   weights.  The
 //   nodes of a set are padded to a multiple of 4 with zero rows.  Wavefront t
 //   < 3 evaluates axis t for the batch's nodes (lane n: node n), wavefront 3
 //   clears the padding rows.
@@ -56,9 +57,14 @@ template <int W> struct MfmaSpread {
     static constexpr int NLO = P > 16 ? P - 16 : 0;     // columns past the two 8-column halves
     static constexpr int NLOB = (P * NLO * 2 + 15) / 16;   // their 16-column blocks
     static constexpr int LB = (NLOB + NW - 1) / NW;     // of those, per wavefront (at most)
-    static constexpr int NB = 64;                       // nodes per batch
+    static constexpr int NB = 32;                       // nodes per batch
     static constexpr int NPOS = NB + 8;                 // batch rows: each set padded to 4 nodes
     static constexpr int RS = 20;                       // row / column weight table stride
+    // one table row per batch position: plane weights [0, 16), row weights times the
+    // coefficient (re, im) from O_B1, column weights from O_B2; 80 floats, so the
+    // four rows one step reads start 16 banks apart
+    static constexpr int O_B1 = 16, O_B2 = O_B1 + 2 * RS, ROW = 80;
+    static_assert(O_B2 + RS <= ROW, "a table row holds its four parts");
     static_assert(W >= 2 && 2 * W + T <= 32, "every window start fits one of the two plane sets");
     static_assert(P <= RS, "a padded row fits the weight table stride");
 };
@@ -93,7 +99,7 @@ template <typename L> __device__ __forceinline__ MfmaSlotCol mfma_slot_col(int g
 // WIN: the window (inline, no calls: the accumulators stay in registers
 // through the weight evaluation).
 template <int W, int WIN>
-__global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<float> *__restrict__ desc,
+__global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<float> *__restrict__ desc,
                                                           const cplx_t<float> *__restrict__ fs,
                                                           const Chunk *__restrict__ chunks,
                                                           uint32_t *__restrict__ n_chunks,
@@ -104,9 +110,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<floa
     using L = TileLayout<float, 3, W, 4, 0>;   // slot layout (shared with the fold)
     static_assert(L::T0 == S::T && L::T1 == S::T && L::TL == S::T && L::P0 == S::P, "8^3 tiles");
     constexpr int LBN = S::LB > 0 ? S::LB : 1;
-    __shared__ __align__(16) float tA[S::NPOS][16];
-    __shared__ __align__(16) float tB1[S::NPOS][2][S::RS];
-    __shared__ __align__(16) float tB2[S::NPOS][S::RS];
+    __shared__ __align__(16) float tw[S::NPOS][S::ROW];
     __shared__ __align__(16) float stg[2 * L::SC];   // the slot image
     __shared__ uint32_t s_next;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -149,18 +153,26 @@ __global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<floa
     struct Ops {
         float a, w2lo, w2hi, w1[S::RW], x1[LBN], x2[LBN];
     };
+    // per-lane offsets in a table row
+    const int oa = nn, ob2 = S::O_B2 + cc, ob1 = S::O_B1 + ri * S::RS + rb;
+    int ox1[LBN], ox2[LBN];
+#pragma unroll
+    for (int j = 0; j < LBN; ++j) {
+        ox1[j] = S::O_B1 + ri * S::RS + lb_b[j];
+        ox2[j] = S::O_B2 + lb_c[j];
+    }
     auto load_ops = [&](int ks) {
         Ops o;
-        const int p = 4 * ks + g;
-        o.a = tA[p][nn];
-        o.w2lo = tB2[p][cc];
-        o.w2hi = tB2[p][8 + cc];
+        const float *row = &tw[0][0] + (4 * ks + g) * S::ROW;
+        o.a = row[oa];
+        o.w2lo = row[ob2];
+        o.w2hi = row[ob2 + 8];
 #pragma unroll
-        for (int r = 0; r < S::RW; ++r) o.w1[r] = r < nrows ? tB1[p][ri][rb + r] : 0.f;
+        for (int r = 0; r < S::RW; ++r) o.w1[r] = r < nrows ? row[ob1 + r] : 0.f;
 #pragma unroll
         for (int j = 0; j < S::LB; ++j) {
-            o.x1[j] = tB1[p][ri][lb_b[j]];
-            o.x2[j] = tB2[p][lb_c[j]];
+            o.x1[j] = row[ox1[j]];
+            o.x2[j] = row[ox2[j]];
         }
         return o;
     };
@@ -181,14 +193,18 @@ __global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<floa
         }
     };
 
-    // the steps [k0, k1) of one set, the next step's operands in flight
+    // the steps [k0, k1) of one set, the next step's operands in flight (two
+    // operand sets in turn, so no copies between steps)
     auto run_set = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
         if (k0 >= k1) return;
-        Ops cur = load_ops(k0);
-        for (int ks = k0; ks < k1; ++ks) {
-            const Ops nxt = load_ops(ks + 1 < k1 ? ks + 1 : ks);
-            mma(cur, acc, accL);
-            cur = nxt;
+        Ops o0 = load_ops(k0);
+        for (int ks = k0;; ks += 2) {
+            const Ops o1 = load_ops(ks + 1 < k1 ? ks + 1 : ks);
+            mma(o0, acc, accL);
+            if (ks + 1 >= k1) break;
+            o0 = load_ops(ks + 2 < k1 ? ks + 2 : ks + 1);
+            mma(o1, acc, accL);
+            if (ks + 2 >= k1) break;
         }
     };
 
@@ -224,7 +240,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<floa
                     // cell x of the table row has tap l = x - lo (weight zero outside [0, W))
                     const int lo = t == 0 ? lo0 - (s1 ? S::OFF1 : 0) : (t == 1 ? lo1 : loL);
                     const int nx = t == 0 ? 16 : S::RS;
-                    float *row = t == 0 ? &tA[pos][0] : (t == 1 ? &tB1[pos][0][0] : &tB2[pos][0]);
+                    float *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
                     const float sc0 = t == 1 ? fv.x : 1.f, sc1 = fv.y;
 #pragma unroll 1
                     for (int x0 = 0; x0 < nx; x0 += 4) {
@@ -248,16 +264,12 @@ __global__ __launch_bounds__(256, 2) void k_tile_spread_mfma(const NodeDesc<floa
                 }
             } else {
                 // zero rows for the padding positions [n0, p0pad) and [p0pad + n1, npos)
-                constexpr int U = 4 + 2 * S::RS / 4 + S::RS / 4;   // 16-byte units per position
+                constexpr int U = S::ROW / 4;   // 16-byte units per row
                 const int pad0 = p0pad - n0, npad = npos - n0 - n1;
                 for (int i = lane; i < npad * U; i += 64) {
                     const int pi = i / U, u = i - pi * U;
                     const int pp = pi < pad0 ? n0 + pi : p0pad + n1 + (pi - pad0);
-                    float4 *d = u < 4 ? reinterpret_cast<float4 *>(&tA[pp][0]) + u
-                              : u < 4 + 2 * S::RS / 4
-                                  ? reinterpret_cast<float4 *>(&tB1[pp][0][0]) + (u - 4)
-                                  : reinterpret_cast<float4 *>(&tB2[pp][0]) + (u - 4 - 2 * S::RS / 4);
-                    *d = make_float4(0.f, 0.f, 0.f, 0.f);
+                    reinterpret_cast<float4 *>(&tw[pp][0])[u] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
             }
             __syncthreads();

@@ -25,11 +25,12 @@
 //   blocks of 16 columns (columns c in [0, 8) and [8, 16), re/im), and the
 //   columns c >= 16 of all rows (P - 16 of them) are packed into extra
 //   blocks spread over the wavefronts.
-// * Per batch of 64 nodes the weights go to LDS once, in padded-tile
+// * Per batch of 32 nodes the weights go to LDS once, in padded-tile
 //   coordinates (zero outside the window), one table row per node: plane
 //   weights per set row, row weights times the coefficient (re and im), column
-//   weights.  The
-//   nodes of a set are padded to a multiple of 4 with zero rows.  Wavefront t
+//   weights.  The nodes of a set are padded to a multiple of 4 with zero
+//   rows.  The coefficients are read through the node permutation (no sorted
+//   copy), one batch ahead.  Wavefront t
 //   < 3 evaluates axis t for the batch's nodes (lane n: node n), wavefront 3
 //   clears the padding rows.
 // * The accumulators go to an LDS image of the chunk's slot (the fold's
@@ -100,7 +101,8 @@ template <typename L> __device__ __forceinline__ MfmaSlotCol mfma_slot_col(int g
 // through the weight evaluation).
 template <int W, int WIN>
 __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<float> *__restrict__ desc,
-                                                          const cplx_t<float> *__restrict__ fs,
+                                                          const uint32_t *__restrict__ perm,
+                                                          const cplx_t<float> *__restrict__ f,
                                                           const Chunk *__restrict__ chunks,
                                                           uint32_t *__restrict__ n_chunks,
                                                           cplx_t<float> *__restrict__ slots,
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
         ch = chunks[c];
         if ((uint32_t)lane < ch.count) {
             nd = desc[ch.start + lane];
-            fv = fs[ch.start + lane];
+            fv = f[perm[ch.start + lane]];
         }
     }
 
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 const uint32_t sn = s0 + S::NB + (uint32_t)lane;
                 if (sn < ch.count) {
                     nd = desc[ch.start + sn];
-                    fv = fs[ch.start + sn];
+                    fv = f[perm[ch.start + sn]];
                 }
             }
             run_set(0, p0pad >> 2, acc0, accL0);
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             ch = chunks[c];
             if ((uint32_t)lane < ch.count) {
                 nd = desc[ch.start + lane];
-                fv = fs[ch.start + lane];
+                fv = f[perm[ch.start + lane]];
             }
         }
         // the next chunk writes the image only after its first batch's barrier,

@@ -24,9 +24,12 @@
 // Adjoint: the accumulated padded tile of each chunk is written to a slot in
 // a destination-blocked order, and k_tile_fold sums, for every grid tile, the
 // slot blocks of the chunks whose padded tiles cover it: no atomics, no zero
-// fill, and a result that does not depend on scheduling.
+// fill, and a result that does not depend on scheduling.  The FP32 3D
+// adjoint on 8^3 tiles accumulates its slots on the matrix cores instead
+// (grid_mfma.hpp); the fold is shared.
 #pragma once
-#include <atomic>
+#include <map>
+#include <mutex>
 
 #include "bin.hpp"
 
@@ -2035,18 +2038,26 @@ static __global__ void k_clear_u32(uint32_t *p)
 
 // Workgroups of kernel k (threads per workgroup, no dynamic LDS) that the
 // current device holds at once: its CUs times the occupancy per CU (asked
-// once per device and kernel).
+// once per device and kernel; kernels of one signature differ in occupancy,
+// so the cache is keyed by the kernel itself).
 template <typename K> static int64_t resident_blocks(K k, int threads)
 {
-    static std::atomic<int64_t> known[64] = {};
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, int64_t> known;
     int dev = 0;
     CUNFFT_HIP(hipGetDevice(&dev));
-    if (dev >= 0 && dev < 64 && known[dev].load() > 0) return known[dev].load();
+    const std::pair<int, const void *> key{dev, reinterpret_cast<const void *>(k)};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        const auto it = known.find(key);
+        if (it != known.end()) return it->second;
+    }
     int cus = 0, per = 0;
     CUNFFT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     CUNFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, 0));
     const int64_t r = (int64_t)std::max(1, cus) * std::max(1, per);
-    if (dev >= 0 && dev < 64) known[dev].store(r);
+    std::lock_guard<std::mutex> lk(mu);
+    known[key] = r;
     return r;
 }
 
@@ -2067,7 +2078,7 @@ template <typename T> struct SpreadOp {
             // the FP32 3D spreads read f through the permutation themselves
             constexpr bool MFMA = sizeof(T) == 4 && D == 3 && V == 0 && kMfmaSpread<W>;
             constexpr bool QUEUE = D == 3 && L::G == 1;
-            constexpr bool PERM_F = QUEUE && !MFMA && sizeof(T) == 4;
+            constexpr bool PERM_F = (QUEUE || MFMA) && sizeof(T) == 4;
             if (!PERM_F || b.compensated)
                 k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
             constexpr int CORE = L::T0 * L::T1 * L::TL;
@@ -2094,8 +2105,8 @@ template <typename T> struct SpreadOp {
                                                       : k_tile_spread_mfma<W, WIN_GAUSSIAN>;
                 k_clear_u32<<<1, 64, 0, s>>>(b.n_chunks + 1);
                 const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(km, 256));
-                km<<<grid, 256, 0, s>>>(reinterpret_cast<const NodeDesc<float> *>(b.desc),
-                                        reinterpret_cast<const cplx_t<float> *>(fs), b.chunks, b.n_chunks,
+                km<<<grid, 256, 0, s>>>(reinterpret_cast<const NodeDesc<float> *>(b.desc), b.perm,
+                                        reinterpret_cast<const cplx_t<float> *>(f), b.chunks, b.n_chunks,
                                         reinterpret_cast<cplx_t<float> *>(slots), win_const<float>(ge));
             } else if constexpr (QUEUE) {
                 // persistent held-unit spread: clear the work counter (a kernel, not a

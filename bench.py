@@ -35,6 +35,7 @@ NN3 = [256, 256, 256]
 M_NODES = 2 ** 21
 CUT = 4
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
 
 
 def log(*a):
@@ -548,8 +549,10 @@ def main():
     shapes = ["8x8x8", "8x8x16", "16x16x8"]
     names = {"fwd_gather_ms": f"k_tile_gather<float,3,10,NW,{cf}> ({shapes[cf]} tiles, forward "
                               "gridding)" if cf < 3 else "k_gather_tiled<float,3,10> (node lanes)",
-             "adj_scatter_ms": f"k_tile_spread<float,3,10,NW,{ca}> ({shapes[ca]} tiles, adjoint "
-                               "tile accumulation, with the sorted-coefficient copy)"
+             "adj_scatter_ms": "k_tile_spread_mfma<10,0> (8x8x8 tiles, adjoint tile accumulation on "
+                               "the matrix cores)" if ca == 0
+                               else f"k_tile_spread_queue<float,3,10,4,{ca}> ({shapes[ca]} tiles, "
+                               "adjoint tile accumulation)"
                                if ca < 3 else "k_scatter_tiled<float,double,3,10> (node lanes)"}
     kern = max(names, key=lambda k: t32[k])
     ach = grid_bytes(8) / (t32[kern] * 1e-3) / 1e9
@@ -562,6 +565,14 @@ def main():
                               "(profiles/traffic_gridding_3d128_fp32.txt); null when that "
                               "summary does not cover the configuration the autotuner chose",
             "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
+    if kern == "adj_scatter_ms" and ca == 0:
+        # the spread's products on the matrix cores: the useful FP32 flops (two FMAs per
+        # window cell and node) against the dense FP32 MFMA peak
+        useful = M_NODES * (2 * CUT + 2) ** 3 * 4
+        roof["matrix_cores"] = {"useful_flops": useful,
+                                "useful_tflops": useful / (t32[kern] * 1e-3) / 1e12,
+                                "peak_tflops": MFMA_FP32_PEAK_TFLOPS,
+                                "frac": useful / (t32[kern] * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS}
     cpu = None
     if not a.no_cpu_baseline:
         log("bench: CPU baseline")

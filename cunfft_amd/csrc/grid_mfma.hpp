@@ -30,9 +30,9 @@
 //   weights per set row, row weights times the coefficient (re and im), column
 //   weights.  The nodes of a set are padded to a multiple of 4 with zero
 //   rows.  The coefficients are read through the node permutation (no sorted
-//   copy), one batch ahead.  Wavefront t
-//   < 3 evaluates axis t for the batch's nodes (lane n: node n), wavefront 3
-//   clears the padding rows.
+//   copy), one batch ahead.  Wavefront t < 3 evaluates axis t for the batch's
+//   nodes (lanes n and n + 32: node n, each half of the row's cells),
+//   wavefront 3 clears the padding rows.
 // * The accumulators go to an LDS image of the chunk's slot (the fold's
 //   destination-blocked order), which the workgroup then copies out with
 //   16-byte stores: whole cache lines, where storing the accumulators
@@ -139,15 +139,18 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
     if (tid == 0) s_next = atomicAdd(counter, 1u);
     __syncthreads();
     uint32_t c = s_next;
-    // lane n < 64: node n of the next batch to run
+    // lanes n and n + 32: node n of the next batch to run (the two halves of a
+    // wavefront share the weight evaluation of a node)
+    static_assert(S::NB == 32, "a batch fills half a wavefront");
+    const int nl = lane & (S::NB - 1), half = lane >> 5;
     NodeDesc<float> nd{};
     cplx_t<float> fv = cmake<float>(0.f, 0.f);
     Chunk ch{0u, 0u, 0u, 0u};
     if (c < n) {
         ch = chunks[c];
-        if ((uint32_t)lane < ch.count) {
-            nd = desc[ch.start + lane];
-            fv = f[perm[ch.start + lane]];
+        if ((uint32_t)nl < ch.count) {
+            nd = desc[ch.start + nl];
+            fv = f[perm[ch.start + nl]];
         }
     }
 
@@ -221,16 +224,16 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             const int nb = __builtin_amdgcn_readfirstlane(
                 ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB);
             if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch's tables
-            const bool valid = lane < nb;
+            const bool valid = nl < nb;
             const uint32_t loc = nd.loc;
             const int lo0 = loc & 255, lo1 = (loc >> 8) & 255, loL = (loc >> 16) & 255;
             const bool s1 = S::SETS2 && valid && lo0 > S::LO_MAX0;
-            const uint64_t m1 = __ballot(s1), m0 = __ballot(valid && !s1);
+            // set masks over the lower half; rank = the node's place in its set
+            const uint64_t m1 = __ballot(s1 && half == 0), m0 = __ballot(valid && !s1 && half == 0);
             const int n0 = __popcll(m0), n1 = __popcll(m1);
             const int p0pad = (n0 + 3) & ~3, npos = p0pad + ((n1 + 3) & ~3);
             const uint64_t mine = s1 ? m1 : m0;
-            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+            const int rank = __popcll(mine & ((1ull << nl) - 1ull));
             const int pos = s1 ? p0pad + rank : rank;
             if (wave < 3) {
                 if (valid) {
@@ -244,8 +247,9 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                     const int nx = t == 0 ? 16 : S::RS;
                     float *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
                     const float sc0 = t == 1 ? fv.x : 1.f, sc1 = fv.y;
+                    // half h of the wavefront writes the quads x0 = 4h, 4h + 8, ...
 #pragma unroll 1
-                    for (int x0 = 0; x0 < nx; x0 += 4) {
+                    for (int x0 = 4 * half; x0 < nx; x0 += 8) {
                         float v[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             __syncthreads();
             // the next batch of this chunk loads during this one's steps
             if (s0 + S::NB < ch.count) {
-                const uint32_t sn = s0 + S::NB + (uint32_t)lane;
+                const uint32_t sn = s0 + S::NB + (uint32_t)nl;
                 if (sn < ch.count) {
                     nd = desc[ch.start + sn];
                     fv = f[perm[ch.start + sn]];
@@ -329,9 +333,9 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
         // the next chunk's first batch loads during the copy-out
         if (c < n) {
             ch = chunks[c];
-            if ((uint32_t)lane < ch.count) {
-                nd = desc[ch.start + lane];
-                fv = f[perm[ch.start + lane]];
+            if ((uint32_t)nl < ch.count) {
+                nd = desc[ch.start + nl];
+                fv = f[perm[ch.start + nl]];
             }
         }
         // the next chunk writes the image only after its first batch's barrier,

@@ -29,8 +29,8 @@
 //   coordinates (zero outside the window), one table row per node: plane
 //   weights per set row, row weights times the coefficient (re and im), column
 //   weights.  The nodes of a set are padded to a multiple of 4 with zero
-//   rows.  The coefficients are read through the node permutation (no sorted
-//   copy), one batch ahead.  Wavefront t < 3 evaluates axis t for the batch's
+//   rows.  Descriptors and coefficients (the sorted copy, so no load waits
+//   on another) load one batch ahead.  Wavefront t < 3 evaluates axis t for the batch's
 //   nodes (lanes n and n + 32: node n, each half of the row's cells),
 //   wavefront 3 clears the padding rows.
 // * The accumulators go to an LDS image of the chunk's slot (the fold's
@@ -66,6 +66,22 @@ template <int W> struct MfmaSpread {
     // four rows one step reads start 16 banks apart
     static constexpr int O_B1 = 16, O_B2 = O_B1 + 2 * RS, ROW = 80;
     static_assert(O_B2 + RS <= ROW, "a table row holds its four parts");
+    // wavefront w: rows [row0(w), row0(w) + rows(w)) and extra(w) extra blocks
+    static constexpr int row0(int w) { return w * P / NW; }
+    static constexpr int rows(int w) { return (w + 1) * P / NW - w * P / NW; }
+    static constexpr int extra(int w)
+    {
+        int k = 0;
+        for (int j = 0; j < LB; ++j) k += w + NW * j < NLOB ? 1 : 0;
+        return k;
+    }
+    // some wavefront has nr rows and nl extra blocks (at most two such shapes)
+    static constexpr bool shape(int nr, int nl)
+    {
+        bool any = false;
+        for (int w = 0; w < NW; ++w) any = any || (rows(w) == nr && extra(w) == nl);
+        return any;
+    }
     static_assert(W >= 2 && 2 * W + T <= 32, "every window start fits one of the two plane sets");
     static_assert(P <= RS, "a padded row fits the weight table stride");
 };
@@ -93,16 +109,19 @@ template <typename L> __device__ __forceinline__ MfmaSlotCol mfma_slot_col(int g
 
 // Persistent: a grid of resident workgroups takes chunks from the work
 // counter n_chunks[1] (zero at launch) until it runs past the chunk count
-// n_chunks[0]; every workgroup takes one ticket past the last chunk and
-// exits.  The next batch's nodes (or the next chunk's first batch) load
-// while the current one runs through the matrix cores, and the operands of
-// the next 4-node step load while the current step's instructions issue.
+// n_chunks[0].  A workgroup holds the tickets of its current and its next
+// chunk and takes a third while the current one runs, so the next chunk's
+// record is in registers when its first batch is loaded; it exits at its
+// first ticket past the last chunk (its tickets rise, so every chunk below
+// the count is run).  The next batch's nodes (or the next chunk's first
+// batch) load while the current one runs through the matrix cores, and the
+// operands of the next 4-node step load while the current step's
+// instructions issue.
 // WIN: the window (inline, no calls: the accumulators stay in registers
 // through the weight evaluation).
 template <int W, int WIN>
 __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<float> *__restrict__ desc,
-                                                          const uint32_t *__restrict__ perm,
-                                                          const cplx_t<float> *__restrict__ f,
+                                                          const cplx_t<float> *__restrict__ fs,
                                                           const Chunk *__restrict__ chunks,
                                                           uint32_t *__restrict__ n_chunks,
                                                           cplx_t<float> *__restrict__ slots,
@@ -114,7 +133,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
     constexpr int LBN = S::LB > 0 ? S::LB : 1;
     __shared__ __align__(16) float tw[S::NPOS][S::ROW];
     __shared__ __align__(16) float stg[2 * L::SC];   // the slot image
-    __shared__ uint32_t s_next;
+    __shared__ uint32_t s_tk[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // operand lanes: A[m = lane % 16][k = lane / 16], B[k = lane / 16][n = lane % 16],
@@ -136,83 +155,97 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
     }
     uint32_t *counter = n_chunks + 1;
     const uint32_t n = n_chunks[0];
-    if (tid == 0) s_next = atomicAdd(counter, 1u);
+    if (tid == 0) {
+        s_tk[0] = atomicAdd(counter, 1u);
+        s_tk[1] = atomicAdd(counter, 1u);
+    }
     __syncthreads();
-    uint32_t c = s_next;
+    uint32_t c = s_tk[0], cn = s_tk[1];
     // lanes n and n + 32: node n of the next batch to run (the two halves of a
     // wavefront share the weight evaluation of a node)
     static_assert(S::NB == 32, "a batch fills half a wavefront");
     const int nl = lane & (S::NB - 1), half = lane >> 5;
     NodeDesc<float> nd{};
     cplx_t<float> fv = cmake<float>(0.f, 0.f);
-    Chunk ch{0u, 0u, 0u, 0u};
+    Chunk ch{0u, 0u, 0u, 0u}, chn{0u, 0u, 0u, 0u};
+    if (cn < n) chn = chunks[cn];
     if (c < n) {
         ch = chunks[c];
         if ((uint32_t)nl < ch.count) {
             nd = desc[ch.start + nl];
-            fv = f[perm[ch.start + nl]];
+            fv = fs[ch.start + nl];
         }
     }
 
-    // operands of one 4-node step
-    struct Ops {
-        float a, w2lo, w2hi, w1[S::RW], x1[LBN], x2[LBN];
-    };
     // per-lane offsets in a table row
-    const int oa = nn, ob2 = S::O_B2 + cc, ob1 = S::O_B1 + ri * S::RS + rb;
+    const int oa = nn, ob2 = S::O_B2 + cc, ob1 = S::O_B1 + ri * S::RS;
     int ox1[LBN], ox2[LBN];
 #pragma unroll
     for (int j = 0; j < LBN; ++j) {
         ox1[j] = S::O_B1 + ri * S::RS + lb_b[j];
         ox2[j] = S::O_B2 + lb_c[j];
     }
-    auto load_ops = [&](int ks) {
-        Ops o;
-        const float *row = &tw[0][0] + (4 * ks + g) * S::ROW;
-        o.a = row[oa];
-        o.w2lo = row[ob2];
-        o.w2hi = row[ob2 + 8];
-#pragma unroll
-        for (int r = 0; r < S::RW; ++r) o.w1[r] = r < nrows ? row[ob1 + r] : 0.f;
-#pragma unroll
-        for (int j = 0; j < S::LB; ++j) {
-            o.x1[j] = row[ox1[j]];
-            o.x2[j] = row[ox2[j]];
-        }
-        return o;
-    };
+    const float *rowg = &tw[0][0] + g * S::ROW;   // step ks reads row 4 ks + g
+    const float *rowb = rowg + ob1 + rb;          // the wavefront's first row weight
     const mf4_t z4 = {0.f, 0.f, 0.f, 0.f};
+
+    // The chunk loop of a wavefront with NR rows and NL extra blocks, both
+    // fixed at compile time (one instance per wavefront shape, at most two per
+    // window width): no branches between the loads and the instructions of a
+    // 4-node step, and the accumulators of one instance only.
+    auto body = [&](auto nr_c, auto nl_c) {
+    constexpr int NR = decltype(nr_c)::value, NL = decltype(nl_c)::value;
     mf4_t acc0[S::RW][2], acc1[S::RW][2], accL0[LBN], accL1[LBN];
-    auto mma = [&](const Ops &o, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
-#pragma unroll
-        for (int r = 0; r < S::RW; ++r) {
-            if (r >= nrows) break;   // wavefront-uniform
-            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2lo, acc[r][0], 0, 0, 0);
-            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2hi, acc[r][1], 0, 0, 0);
-        }
-#pragma unroll
-        for (int j = 0; j < S::LB; ++j) {
-            if (wave + S::NW * j >= S::NLOB) break;   // wavefront-uniform
-            const float bv = lb_ok[j] ? o.x1[j] * o.x2[j] : 0.f;
-            accL[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, bv, accL[j], 0, 0, 0);
-        }
-    };
 
-    // the steps [k0, k1) of one set, the next step's operands in flight (two
-    // operand sets in turn, so no copies between steps)
-    auto run_set = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
-        if (k0 >= k1) return;
-        Ops o0 = load_ops(k0);
-        for (int ks = k0;; ks += 2) {
-            const Ops o1 = load_ops(ks + 1 < k1 ? ks + 1 : ks);
-            mma(o0, acc, accL);
-            if (ks + 1 >= k1) break;
-            o0 = load_ops(ks + 2 < k1 ? ks + 2 : ks + 1);
-            mma(o1, acc, accL);
-            if (ks + 2 >= k1) break;
-        }
+    // The 4-node steps [0, ks0) into set 0 and [ks0, ks1) into set 1; the
+    // next step's operands load while the current step's MFMAs issue (two
+    // operand sets in turn, so no copies between steps).
+    auto ksteps = [&](int ks0, int ks1) {
+        struct Ops {
+            float a, w2lo, w2hi, w1[NR], x1[NL > 0 ? NL : 1], x2[NL > 0 ? NL : 1];
+        };
+        auto load = [&](int ks) {
+            Ops o;
+            const float *row = rowg + ks * (4 * S::ROW);
+            o.a = row[oa];
+            o.w2lo = row[ob2];
+            o.w2hi = row[ob2 + 8];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) o.w1[r] = rowb[ks * (4 * S::ROW) + r];
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                o.x1[j] = row[ox1[j]];
+                o.x2[j] = row[ox2[j]];
+            }
+            return o;
+        };
+        auto mma = [&](const Ops &o, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2lo, acc[r][0], 0, 0, 0);
+                acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2hi, acc[r][1], 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                const float bv = lb_ok[j] ? o.x1[j] * o.x2[j] : 0.f;
+                accL[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, bv, accL[j], 0, 0, 0);
+            }
+        };
+        auto run = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+            if (k0 >= k1) return;
+            Ops o0 = load(k0);
+            for (int ks = k0;; ks += 2) {
+                const Ops o1 = load(ks + 1 < k1 ? ks + 1 : ks);
+                mma(o0, acc, accL);
+                if (ks + 1 >= k1) break;
+                o0 = load(ks + 2 < k1 ? ks + 2 : ks + 1);
+                mma(o1, acc, accL);
+                if (ks + 2 >= k1) break;
+            }
+        };
+        run(0, ks0, acc0, accL0);
+        if constexpr (S::SETS2) run(ks0, ks1, acc1, accL1);
     };
-
     uint32_t ticket = 0;
     while (c < n) {
 #pragma unroll
@@ -225,7 +258,12 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB);
             if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch's tables
             const bool valid = nl < nb;
-            const uint32_t loc = nd.loc;
+            // the batch's node, loaded one batch ahead: opaque from here on, so
+            // no use of it is moved up to its load (which would wait on the
+            // load there, before the previous batch's steps)
+            uint32_t loc = nd.loc;
+            float nf0 = nd.f[0], nf1 = nd.f[1], nf2 = nd.f[2], fre = fv.x, fim = fv.y;
+            asm volatile("" : "+v"(loc), "+v"(nf0), "+v"(nf1), "+v"(nf2), "+v"(fre), "+v"(fim));
             const int lo0 = loc & 255, lo1 = (loc >> 8) & 255, loL = (loc >> 16) & 255;
             const bool s1 = S::SETS2 && valid && lo0 > S::LO_MAX0;
             // set masks over the lower half; rank = the node's place in its set
@@ -238,7 +276,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             if (wave < 3) {
                 if (valid) {
                     const int t = wave;   // axis
-                    const float ft = t == 0 ? nd.f[0] : (t == 1 ? nd.f[1] : nd.f[2]);
+                    const float ft = t == 0 ? nf0 : (t == 1 ? nf1 : nf2);
                     const float kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
                     const float nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
                     const float shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
@@ -246,7 +284,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                     const int lo = t == 0 ? lo0 - (s1 ? S::OFF1 : 0) : (t == 1 ? lo1 : loL);
                     const int nx = t == 0 ? 16 : S::RS;
                     float *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
-                    const float sc0 = t == 1 ? fv.x : 1.f, sc1 = fv.y;
+                    const float sc0 = t == 1 ? fre : 1.f, sc1 = fim;
                     // half h of the wavefront writes the quads x0 = 4h, 4h + 8, ...
 #pragma unroll 1
                     for (int x0 = 4 * half; x0 < nx; x0 += 8) {
@@ -284,11 +322,10 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 const uint32_t sn = s0 + S::NB + (uint32_t)nl;
                 if (sn < ch.count) {
                     nd = desc[ch.start + sn];
-                    fv = f[perm[ch.start + sn]];
+                    fv = fs[ch.start + sn];
                 }
             }
-            run_set(0, p0pad >> 2, acc0, accL0);
-            if constexpr (S::SETS2) run_set(p0pad >> 2, npos >> 2, acc1, accL1);
+            ksteps(p0pad >> 2, S::SETS2 ? npos >> 2 : p0pad >> 2);
         }
         // the sum of the two sets per plane into the slot image; lane group g holds
         // planes 4g + i of set 0 and 4g + i + OFF1 of set 1
@@ -316,34 +353,47 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             }
         };
 #pragma unroll
-        for (int r = 0; r < S::RW; ++r) {
-            if (r >= nrows) break;
+        for (int r = 0; r < NR; ++r) {
             put(acc0[r][0], acc1[r][0], rb + r, cc, cc < S::P);
             put(acc0[r][1], acc1[r][1], rb + r, 8 + cc, 8 + cc < S::P);
         }
 #pragma unroll
-        for (int j = 0; j < S::LB; ++j) {
-            if (wave + S::NW * j >= S::NLOB) break;
-            put(accL0[j], accL1[j], lb_b[j], lb_c[j], lb_ok[j]);
-        }
-        if (tid == 0) s_next = ticket;
+        for (int j = 0; j < NL; ++j) put(accL0[j], accL1[j], lb_b[j], lb_c[j], lb_ok[j]);
+        if (tid == 0) s_tk[0] = ticket;
         __syncthreads();   // the image is whole and the ticket visible
         const uint32_t cur_c = c;
-        c = s_next;
-        // the next chunk's first batch loads during the copy-out
-        if (c < n) {
-            ch = chunks[c];
-            if ((uint32_t)nl < ch.count) {
-                nd = desc[ch.start + nl];
-                fv = f[perm[ch.start + nl]];
-            }
+        c = cn;
+        ch = chn;
+        cn = s_tk[0];
+        // the next chunk's first batch loads during the copy-out, and the record
+        // of the chunk after it during the next chunk's steps
+        if (c < n && (uint32_t)nl < ch.count) {
+            nd = desc[ch.start + nl];
+            fv = fs[ch.start + nl];
         }
+        if (cn < n) chn = chunks[cn];
         // the next chunk writes the image only after its first batch's barrier,
         // which every wavefront reaches after its copy-out reads
         float4 *dst = reinterpret_cast<float4 *>(slots + (int64_t)cur_c * L::SC);
         const float4 *src = reinterpret_cast<const float4 *>(stg);
         for (int i = tid; i < L::SC / 2; i += 256) stream_store(dst + i, src[i]);
     }
+    };
+
+    using std::integral_constant;
+    constexpr int RLO = S::P / S::NW, LLO = S::NLOB / S::NW;
+    const int nlb = S::extra(wave);
+    if constexpr (S::shape(RLO, LLO))
+        if (nrows == RLO && nlb == LLO) return body(integral_constant<int, RLO>{}, integral_constant<int, LLO>{});
+    if constexpr (S::shape(RLO, LLO + 1))
+        if (nrows == RLO && nlb == LLO + 1)
+            return body(integral_constant<int, RLO>{}, integral_constant<int, LLO + 1>{});
+    if constexpr (S::shape(RLO + 1, LLO))
+        if (nrows == RLO + 1 && nlb == LLO)
+            return body(integral_constant<int, RLO + 1>{}, integral_constant<int, LLO>{});
+    if constexpr (S::shape(RLO + 1, LLO + 1))
+        if (nrows == RLO + 1 && nlb == LLO + 1)
+            return body(integral_constant<int, RLO + 1>{}, integral_constant<int, LLO + 1>{});
 }
 
 }  // namespace cunfft

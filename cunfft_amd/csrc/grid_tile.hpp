@@ -2078,7 +2078,7 @@ template <typename T> struct SpreadOp {
             // the FP32 3D spreads read f through the permutation themselves
             constexpr bool MFMA = sizeof(T) == 4 && D == 3 && V == 0 && kMfmaSpread<W>;
             constexpr bool QUEUE = D == 3 && L::G == 1;
-            constexpr bool PERM_F = (QUEUE || MFMA) && sizeof(T) == 4;
+            constexpr bool PERM_F = QUEUE && !MFMA && sizeof(T) == 4;
             if (!PERM_F || b.compensated)
                 k_sorted_coeffs<T><<<(unsigned)((ge.M + 255) / 256), 256, 0, s>>>(f, b.perm, fs, ge.M);
             constexpr int CORE = L::T0 * L::T1 * L::TL;
@@ -2105,8 +2105,8 @@ template <typename T> struct SpreadOp {
                                                       : k_tile_spread_mfma<W, WIN_GAUSSIAN>;
                 k_clear_u32<<<1, 64, 0, s>>>(b.n_chunks + 1);
                 const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(km, 256));
-                km<<<grid, 256, 0, s>>>(reinterpret_cast<const NodeDesc<float> *>(b.desc), b.perm,
-                                        reinterpret_cast<const cplx_t<float> *>(f), b.chunks, b.n_chunks,
+                km<<<grid, 256, 0, s>>>(reinterpret_cast<const NodeDesc<float> *>(b.desc),
+                                        reinterpret_cast<const cplx_t<float> *>(fs), b.chunks, b.n_chunks,
                                         reinterpret_cast<cplx_t<float> *>(slots), win_const<float>(ge));
             } else if constexpr (QUEUE) {
                 // persistent held-unit spread: clear the work counter (a kernel, not a

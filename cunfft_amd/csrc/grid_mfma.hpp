@@ -75,6 +75,13 @@ template <int W> struct MfmaSpread {
         for (int j = 0; j < LB; ++j) k += w + NW * j < NLOB ? 1 : 0;
         return k;
     }
+    // every wavefront's first row weight is 16-byte aligned in its table row
+    static constexpr bool rows_aligned()
+    {
+        bool ok = (O_B1 % 4 == 0) && (RS % 4 == 0) && (ROW % 4 == 0);
+        for (int w = 0; w < NW; ++w) ok = ok && row0(w) % 4 == 0;
+        return ok;
+    }
     // some wavefront has nr rows and nl extra blocks (at most two such shapes)
     static constexpr bool shape(int nr, int nl)
     {
@@ -210,8 +217,20 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             o.a = row[oa];
             o.w2lo = row[ob2];
             o.w2hi = row[ob2 + 8];
+            const float *w1p = rowb + ks * (4 * S::ROW);
+            if constexpr (S::rows_aligned() && NR >= 4) {
+                // the first four row weights in one 16-byte read
+                const float4 q = *reinterpret_cast<const float4 *>(w1p);
+                o.w1[0] = q.x;
+                o.w1[1] = q.y;
+                o.w1[2] = q.z;
+                o.w1[3] = q.w;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) o.w1[r] = rowb[ks * (4 * S::ROW) + r];
+                for (int r = 4; r < NR; ++r) o.w1[r] = w1p[r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) o.w1[r] = w1p[r];
+            }
 #pragma unroll
             for (int j = 0; j < NL; ++j) {
                 o.x1[j] = row[ox1[j]];
@@ -233,12 +252,17 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
         };
         auto run = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
             if (k0 >= k1) return;
+            // (the scheduling barriers keep each step's loads ahead of the
+            // previous step's instructions, which the scheduler would sink
+            // to their uses)
             Ops o0 = load(k0);
             for (int ks = k0;; ks += 2) {
                 const Ops o1 = load(ks + 1 < k1 ? ks + 1 : ks);
+                __builtin_amdgcn_sched_barrier(0);
                 mma(o0, acc, accL);
                 if (ks + 1 >= k1) break;
                 o0 = load(ks + 2 < k1 ? ks + 2 : ks + 1);
+                __builtin_amdgcn_sched_barrier(0);
                 mma(o1, acc, accL);
                 if (ks + 2 >= k1) break;
             }

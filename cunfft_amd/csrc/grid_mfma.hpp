@@ -211,13 +211,40 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
         struct Ops {
             float a, w2lo, w2hi, w1[NR], x1[NL > 0 ? NL : 1], x2[NL > 0 ? NL : 1];
         };
-        auto load = [&](int ks) {
+        // per-lane table pointers of a step's operands; step ks + 1 is 4 rows on
+        constexpr int STEP = 4 * S::ROW;
+        struct Ptrs {
+            const float *a, *w2, *w1, *x1[NL > 0 ? NL : 1], *x2[NL > 0 ? NL : 1];
+        };
+        auto ptrs = [&](int ks) {
+            Ptrs q;
+            q.a = rowg + ks * STEP + oa;
+            q.w2 = rowg + ks * STEP + ob2;
+            q.w1 = rowb + ks * STEP;
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                q.x1[j] = rowg + ks * STEP + ox1[j];
+                q.x2[j] = rowg + ks * STEP + ox2[j];
+            }
+            return q;
+        };
+        auto advance = [&](Ptrs &q, int steps) {
+            q.a += steps * STEP;
+            q.w2 += steps * STEP;
+            q.w1 += steps * STEP;
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                q.x1[j] += steps * STEP;
+                q.x2[j] += steps * STEP;
+            }
+        };
+        // the operands of the step `d` steps past q's (d a constant: immediate offsets)
+        auto load = [&](const Ptrs &q, int d) {
             Ops o;
-            const float *row = rowg + ks * (4 * S::ROW);
-            o.a = row[oa];
-            o.w2lo = row[ob2];
-            o.w2hi = row[ob2 + 8];
-            const float *w1p = rowb + ks * (4 * S::ROW);
+            o.a = q.a[d * STEP];
+            o.w2lo = q.w2[d * STEP];
+            o.w2hi = q.w2[d * STEP + 8];
+            const float *w1p = q.w1 + d * STEP;
             if constexpr (S::rows_aligned() && NR >= 4) {
                 // the first four row weights in one 16-byte read
                 const float4 q = *reinterpret_cast<const float4 *>(w1p);
@@ -233,8 +260,8 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             }
 #pragma unroll
             for (int j = 0; j < NL; ++j) {
-                o.x1[j] = row[ox1[j]];
-                o.x2[j] = row[ox2[j]];
+                o.x1[j] = q.x1[j][d * STEP];
+                o.x2[j] = q.x2[j][d * STEP];
             }
             return o;
         };
@@ -255,13 +282,19 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             // (the scheduling barriers keep each step's loads ahead of the
             // previous step's instructions, which the scheduler would sink
             // to their uses)
-            Ops o0 = load(k0);
+            // a set's last step reads the operands of one step past it: the two
+            // padded sets fill at most NB + 4 rows (their sizes are multiples of 4
+            // summing to at most NB + 6), so those four rows are inside the table
+            static_assert(S::NB % 4 == 0 && S::NPOS >= S::NB + 8, "one step past the last stays in the table");
+            Ptrs q = ptrs(k0);
+            Ops o0 = load(q, 0);
             for (int ks = k0;; ks += 2) {
-                const Ops o1 = load(ks + 1 < k1 ? ks + 1 : ks);
+                const Ops o1 = load(q, 1);
                 __builtin_amdgcn_sched_barrier(0);
                 mma(o0, acc, accL);
                 if (ks + 1 >= k1) break;
-                o0 = load(ks + 2 < k1 ? ks + 2 : ks + 1);
+                o0 = load(q, 2);
+                advance(q, 2);
                 __builtin_amdgcn_sched_barrier(0);
                 mma(o1, acc, accL);
                 if (ks + 2 >= k1) break;

@@ -201,240 +201,240 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
     // window width): no branches between the loads and the instructions of a
     // 4-node step, and the accumulators of one instance only.
     auto body = [&](auto nr_c, auto nl_c) {
-    constexpr int NR = decltype(nr_c)::value, NL = decltype(nl_c)::value;
-    mf4_t acc0[S::RW][2], acc1[S::RW][2], accL0[LBN], accL1[LBN];
+        constexpr int NR = decltype(nr_c)::value, NL = decltype(nl_c)::value;
+        mf4_t acc0[S::RW][2], acc1[S::RW][2], accL0[LBN], accL1[LBN];
 
-    // The 4-node steps [0, ks0) into set 0 and [ks0, ks1) into set 1; the
-    // next step's operands load while the current step's MFMAs issue (two
-    // operand sets in turn, so no copies between steps).
-    auto ksteps = [&](int ks0, int ks1) {
-        struct Ops {
-            float a, w2lo, w2hi, w1[NR], x1[NL > 0 ? NL : 1], x2[NL > 0 ? NL : 1];
+        // The 4-node steps [0, ks0) into set 0 and [ks0, ks1) into set 1; the
+        // next step's operands load while the current step's MFMAs issue (two
+        // operand sets in turn, so no copies between steps).
+        auto ksteps = [&](int ks0, int ks1) {
+            struct Ops {
+                float a, w2lo, w2hi, w1[NR], x1[NL > 0 ? NL : 1], x2[NL > 0 ? NL : 1];
+            };
+            // per-lane table pointers of a step's operands; step ks + 1 is 4 rows on
+            constexpr int STEP = 4 * S::ROW;
+            struct Ptrs {
+                const float *a, *w2, *w1, *x1[NL > 0 ? NL : 1], *x2[NL > 0 ? NL : 1];
+            };
+            auto ptrs = [&](int ks) {
+                Ptrs q;
+                q.a = rowg + ks * STEP + oa;
+                q.w2 = rowg + ks * STEP + ob2;
+                q.w1 = rowb + ks * STEP;
+#pragma unroll
+                for (int j = 0; j < NL; ++j) {
+                    q.x1[j] = rowg + ks * STEP + ox1[j];
+                    q.x2[j] = rowg + ks * STEP + ox2[j];
+                }
+                return q;
+            };
+            auto advance = [&](Ptrs &q, int steps) {
+                q.a += steps * STEP;
+                q.w2 += steps * STEP;
+                q.w1 += steps * STEP;
+#pragma unroll
+                for (int j = 0; j < NL; ++j) {
+                    q.x1[j] += steps * STEP;
+                    q.x2[j] += steps * STEP;
+                }
+            };
+            // the operands of the step `d` steps past q's (d a constant: immediate offsets)
+            auto load = [&](const Ptrs &q, int d) {
+                Ops o;
+                o.a = q.a[d * STEP];
+                o.w2lo = q.w2[d * STEP];
+                o.w2hi = q.w2[d * STEP + 8];
+                const float *w1p = q.w1 + d * STEP;
+                if constexpr (S::rows_aligned() && NR >= 4) {
+                    // the first four row weights in one 16-byte read
+                    const float4 q = *reinterpret_cast<const float4 *>(w1p);
+                    o.w1[0] = q.x;
+                    o.w1[1] = q.y;
+                    o.w1[2] = q.z;
+                    o.w1[3] = q.w;
+#pragma unroll
+                    for (int r = 4; r < NR; ++r) o.w1[r] = w1p[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) o.w1[r] = w1p[r];
+                }
+#pragma unroll
+                for (int j = 0; j < NL; ++j) {
+                    o.x1[j] = q.x1[j][d * STEP];
+                    o.x2[j] = q.x2[j][d * STEP];
+                }
+                return o;
+            };
+            auto mma = [&](const Ops &o, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2lo, acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2hi, acc[r][1], 0, 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < NL; ++j) {
+                    const float bv = lb_ok[j] ? o.x1[j] * o.x2[j] : 0.f;
+                    accL[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, bv, accL[j], 0, 0, 0);
+                }
+            };
+            auto run = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+                if (k0 >= k1) return;
+                // (the scheduling barriers keep each step's loads ahead of the
+                // previous step's instructions, which the scheduler would sink
+                // to their uses)
+                // a set's last step reads the operands of one step past it: the two
+                // padded sets fill at most NB + 4 rows (their sizes are multiples of 4
+                // summing to at most NB + 6), so those four rows are inside the table
+                static_assert(S::NB % 4 == 0 && S::NPOS >= S::NB + 8, "one step past the last stays in the table");
+                Ptrs q = ptrs(k0);
+                Ops o0 = load(q, 0);
+                for (int ks = k0;; ks += 2) {
+                    const Ops o1 = load(q, 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma(o0, acc, accL);
+                    if (ks + 1 >= k1) break;
+                    o0 = load(q, 2);
+                    advance(q, 2);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma(o1, acc, accL);
+                    if (ks + 2 >= k1) break;
+                }
+            };
+            run(0, ks0, acc0, accL0);
+            if constexpr (S::SETS2) run(ks0, ks1, acc1, accL1);
         };
-        // per-lane table pointers of a step's operands; step ks + 1 is 4 rows on
-        constexpr int STEP = 4 * S::ROW;
-        struct Ptrs {
-            const float *a, *w2, *w1, *x1[NL > 0 ? NL : 1], *x2[NL > 0 ? NL : 1];
-        };
-        auto ptrs = [&](int ks) {
-            Ptrs q;
-            q.a = rowg + ks * STEP + oa;
-            q.w2 = rowg + ks * STEP + ob2;
-            q.w1 = rowb + ks * STEP;
+        uint32_t ticket = 0;
+        while (c < n) {
 #pragma unroll
-            for (int j = 0; j < NL; ++j) {
-                q.x1[j] = rowg + ks * STEP + ox1[j];
-                q.x2[j] = rowg + ks * STEP + ox2[j];
-            }
-            return q;
-        };
-        auto advance = [&](Ptrs &q, int steps) {
-            q.a += steps * STEP;
-            q.w2 += steps * STEP;
-            q.w1 += steps * STEP;
+            for (int r = 0; r < S::RW; ++r) acc0[r][0] = acc0[r][1] = acc1[r][0] = acc1[r][1] = z4;
 #pragma unroll
-            for (int j = 0; j < NL; ++j) {
-                q.x1[j] += steps * STEP;
-                q.x2[j] += steps * STEP;
-            }
-        };
-        // the operands of the step `d` steps past q's (d a constant: immediate offsets)
-        auto load = [&](const Ptrs &q, int d) {
-            Ops o;
-            o.a = q.a[d * STEP];
-            o.w2lo = q.w2[d * STEP];
-            o.w2hi = q.w2[d * STEP + 8];
-            const float *w1p = q.w1 + d * STEP;
-            if constexpr (S::rows_aligned() && NR >= 4) {
-                // the first four row weights in one 16-byte read
-                const float4 q = *reinterpret_cast<const float4 *>(w1p);
-                o.w1[0] = q.x;
-                o.w1[1] = q.y;
-                o.w1[2] = q.z;
-                o.w1[3] = q.w;
-#pragma unroll
-                for (int r = 4; r < NR; ++r) o.w1[r] = w1p[r];
-            } else {
-#pragma unroll
-                for (int r = 0; r < NR; ++r) o.w1[r] = w1p[r];
-            }
-#pragma unroll
-            for (int j = 0; j < NL; ++j) {
-                o.x1[j] = q.x1[j][d * STEP];
-                o.x2[j] = q.x2[j][d * STEP];
-            }
-            return o;
-        };
-        auto mma = [&](const Ops &o, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2lo, acc[r][0], 0, 0, 0);
-                acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2hi, acc[r][1], 0, 0, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < NL; ++j) {
-                const float bv = lb_ok[j] ? o.x1[j] * o.x2[j] : 0.f;
-                accL[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, bv, accL[j], 0, 0, 0);
-            }
-        };
-        auto run = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
-            if (k0 >= k1) return;
-            // (the scheduling barriers keep each step's loads ahead of the
-            // previous step's instructions, which the scheduler would sink
-            // to their uses)
-            // a set's last step reads the operands of one step past it: the two
-            // padded sets fill at most NB + 4 rows (their sizes are multiples of 4
-            // summing to at most NB + 6), so those four rows are inside the table
-            static_assert(S::NB % 4 == 0 && S::NPOS >= S::NB + 8, "one step past the last stays in the table");
-            Ptrs q = ptrs(k0);
-            Ops o0 = load(q, 0);
-            for (int ks = k0;; ks += 2) {
-                const Ops o1 = load(q, 1);
-                __builtin_amdgcn_sched_barrier(0);
-                mma(o0, acc, accL);
-                if (ks + 1 >= k1) break;
-                o0 = load(q, 2);
-                advance(q, 2);
-                __builtin_amdgcn_sched_barrier(0);
-                mma(o1, acc, accL);
-                if (ks + 2 >= k1) break;
-            }
-        };
-        run(0, ks0, acc0, accL0);
-        if constexpr (S::SETS2) run(ks0, ks1, acc1, accL1);
-    };
-    uint32_t ticket = 0;
-    while (c < n) {
-#pragma unroll
-        for (int r = 0; r < S::RW; ++r) acc0[r][0] = acc0[r][1] = acc1[r][0] = acc1[r][1] = z4;
-#pragma unroll
-        for (int j = 0; j < LBN; ++j) accL0[j] = accL1[j] = z4;
-        if (tid == 0) ticket = atomicAdd(counter, 1u);
-        for (uint32_t s0 = 0; s0 < ch.count; s0 += S::NB) {
-            const int nb = __builtin_amdgcn_readfirstlane(
-                ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB);
-            if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch's tables
-            const bool valid = nl < nb;
-            // the batch's node, loaded one batch ahead: opaque from here on, so
-            // no use of it is moved up to its load (which would wait on the
-            // load there, before the previous batch's steps)
-            uint32_t loc = nd.loc;
-            float nf0 = nd.f[0], nf1 = nd.f[1], nf2 = nd.f[2], fre = fv.x, fim = fv.y;
-            asm volatile("" : "+v"(loc), "+v"(nf0), "+v"(nf1), "+v"(nf2), "+v"(fre), "+v"(fim));
-            const int lo0 = loc & 255, lo1 = (loc >> 8) & 255, loL = (loc >> 16) & 255;
-            const bool s1 = S::SETS2 && valid && lo0 > S::LO_MAX0;
-            // set masks over the lower half; rank = the node's place in its set
-            const uint64_t m1 = __ballot(s1 && half == 0), m0 = __ballot(valid && !s1 && half == 0);
-            const int n0 = __popcll(m0), n1 = __popcll(m1);
-            const int p0pad = (n0 + 3) & ~3, npos = p0pad + ((n1 + 3) & ~3);
-            const uint64_t mine = s1 ? m1 : m0;
-            const int rank = __popcll(mine & ((1ull << nl) - 1ull));
-            const int pos = s1 ? p0pad + rank : rank;
-            if (wave < 3) {
-                if (valid) {
-                    const int t = wave;   // axis
-                    const float ft = t == 0 ? nf0 : (t == 1 ? nf1 : nf2);
-                    const float kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
-                    const float nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
-                    const float shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
-                    // cell x of the table row has tap l = x - lo (weight zero outside [0, W))
-                    const int lo = t == 0 ? lo0 - (s1 ? S::OFF1 : 0) : (t == 1 ? lo1 : loL);
-                    const int nx = t == 0 ? 16 : S::RS;
-                    float *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
-                    const float sc0 = t == 1 ? fre : 1.f, sc1 = fim;
-                    // half h of the wavefront writes the quads x0 = 4h, 4h + 8, ...
+            for (int j = 0; j < LBN; ++j) accL0[j] = accL1[j] = z4;
+            if (tid == 0) ticket = atomicAdd(counter, 1u);
+            for (uint32_t s0 = 0; s0 < ch.count; s0 += S::NB) {
+                const int nb = __builtin_amdgcn_readfirstlane(
+                    ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB);
+                if (s0 > 0) __syncthreads();   // every wavefront is done with the previous batch's tables
+                const bool valid = nl < nb;
+                // the batch's node, loaded one batch ahead: opaque from here on, so
+                // no use of it is moved up to its load (which would wait on the
+                // load there, before the previous batch's steps)
+                uint32_t loc = nd.loc;
+                float nf0 = nd.f[0], nf1 = nd.f[1], nf2 = nd.f[2], fre = fv.x, fim = fv.y;
+                asm volatile("" : "+v"(loc), "+v"(nf0), "+v"(nf1), "+v"(nf2), "+v"(fre), "+v"(fim));
+                const int lo0 = loc & 255, lo1 = (loc >> 8) & 255, loL = (loc >> 16) & 255;
+                const bool s1 = S::SETS2 && valid && lo0 > S::LO_MAX0;
+                // set masks over the lower half; rank = the node's place in its set
+                const uint64_t m1 = __ballot(s1 && half == 0), m0 = __ballot(valid && !s1 && half == 0);
+                const int n0 = __popcll(m0), n1 = __popcll(m1);
+                const int p0pad = (n0 + 3) & ~3, npos = p0pad + ((n1 + 3) & ~3);
+                const uint64_t mine = s1 ? m1 : m0;
+                const int rank = __popcll(mine & ((1ull << nl) - 1ull));
+                const int pos = s1 ? p0pad + rank : rank;
+                if (wave < 3) {
+                    if (valid) {
+                        const int t = wave;   // axis
+                        const float ft = t == 0 ? nf0 : (t == 1 ? nf1 : nf2);
+                        const float kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
+                        const float nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
+                        const float shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
+                        // cell x of the table row has tap l = x - lo (weight zero outside [0, W))
+                        const int lo = t == 0 ? lo0 - (s1 ? S::OFF1 : 0) : (t == 1 ? lo1 : loL);
+                        const int nx = t == 0 ? 16 : S::RS;
+                        float *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
+                        const float sc0 = t == 1 ? fre : 1.f, sc1 = fim;
+                        // half h of the wavefront writes the quads x0 = 4h, 4h + 8, ...
 #pragma unroll 1
-                    for (int x0 = 4 * half; x0 < nx; x0 += 8) {
-                        float v[4];
+                        for (int x0 = 4 * half; x0 < nx; x0 += 8) {
+                            float v[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int l = x0 + u - lo;
-                            const float sv = ft + (float)(wc.m - l);
-                            float w;
-                            if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<float>(sv, (float)wc.m, shpt);
-                            else if constexpr (WIN == WIN_SINC) w = sinc_phi<float>(sv, wc.m, shpt);
-                            else w = nrt * wexp<float>(-kbt * sv * sv);
-                            v[u] = l >= 0 && l < W ? w : 0.f;
+                            for (int u = 0; u < 4; ++u) {
+                                const int l = x0 + u - lo;
+                                const float sv = ft + (float)(wc.m - l);
+                                float w;
+                                if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<float>(sv, (float)wc.m, shpt);
+                                else if constexpr (WIN == WIN_SINC) w = sinc_phi<float>(sv, wc.m, shpt);
+                                else w = nrt * wexp<float>(-kbt * sv * sv);
+                                v[u] = l >= 0 && l < W ? w : 0.f;
+                            }
+                            *reinterpret_cast<float4 *>(row + x0) =
+                                make_float4(v[0] * sc0, v[1] * sc0, v[2] * sc0, v[3] * sc0);
+                            if (t == 1)
+                                *reinterpret_cast<float4 *>(row + S::RS + x0) =
+                                    make_float4(v[0] * sc1, v[1] * sc1, v[2] * sc1, v[3] * sc1);
                         }
-                        *reinterpret_cast<float4 *>(row + x0) =
-                            make_float4(v[0] * sc0, v[1] * sc0, v[2] * sc0, v[3] * sc0);
-                        if (t == 1)
-                            *reinterpret_cast<float4 *>(row + S::RS + x0) =
-                                make_float4(v[0] * sc1, v[1] * sc1, v[2] * sc1, v[3] * sc1);
+                    }
+                } else {
+                    // zero rows for the padding positions [n0, p0pad) and [p0pad + n1, npos)
+                    constexpr int U = S::ROW / 4;   // 16-byte units per row
+                    const int pad0 = p0pad - n0, npad = npos - n0 - n1;
+                    for (int i = lane; i < npad * U; i += 64) {
+                        const int pi = i / U, u = i - pi * U;
+                        const int pp = pi < pad0 ? n0 + pi : p0pad + n1 + (pi - pad0);
+                        reinterpret_cast<float4 *>(&tw[pp][0])[u] = make_float4(0.f, 0.f, 0.f, 0.f);
                     }
                 }
-            } else {
-                // zero rows for the padding positions [n0, p0pad) and [p0pad + n1, npos)
-                constexpr int U = S::ROW / 4;   // 16-byte units per row
-                const int pad0 = p0pad - n0, npad = npos - n0 - n1;
-                for (int i = lane; i < npad * U; i += 64) {
-                    const int pi = i / U, u = i - pi * U;
-                    const int pp = pi < pad0 ? n0 + pi : p0pad + n1 + (pi - pad0);
-                    reinterpret_cast<float4 *>(&tw[pp][0])[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                __syncthreads();
+                // the next batch of this chunk loads during this one's steps
+                if (s0 + S::NB < ch.count) {
+                    const uint32_t sn = s0 + S::NB + (uint32_t)nl;
+                    if (sn < ch.count) {
+                        nd = desc[ch.start + sn];
+                        fv = fs[ch.start + sn];
+                    }
                 }
+                ksteps(p0pad >> 2, S::SETS2 ? npos >> 2 : p0pad >> 2);
             }
-            __syncthreads();
-            // the next batch of this chunk loads during this one's steps
-            if (s0 + S::NB < ch.count) {
-                const uint32_t sn = s0 + S::NB + (uint32_t)nl;
-                if (sn < ch.count) {
-                    nd = desc[ch.start + sn];
-                    fv = fs[ch.start + sn];
+            // the sum of the two sets per plane into the slot image; lane group g holds
+            // planes 4g + i of set 0 and 4g + i + OFF1 of set 1
+            auto put = [&](const mf4_t &v0, const mf4_t &v1, int b, int cl, bool ok) {
+                float up[4];   // set 1 at plane 4g + i
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (!S::SETS2) {
+                        up[i] = 0.f;
+                    } else if (i >= S::OFF1) {
+                        up[i] = v1[i - S::OFF1];
+                    } else {
+                        const float t = __shfl(v1[4 - S::OFF1 + i], lane - 16);
+                        up[i] = g > 0 ? t : 0.f;
+                    }
                 }
-            }
-            ksteps(p0pad >> 2, S::SETS2 ? npos >> 2 : p0pad >> 2);
-        }
-        // the sum of the two sets per plane into the slot image; lane group g holds
-        // planes 4g + i of set 0 and 4g + i + OFF1 of set 1
-        auto put = [&](const mf4_t &v0, const mf4_t &v1, int b, int cl, bool ok) {
-            float up[4];   // set 1 at plane 4g + i
+                if (!ok) return;
+                const MfmaSlotCol o = mfma_slot_col<L>(g, b, cl, ri);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (!S::SETS2) {
-                    up[i] = 0.f;
-                } else if (i >= S::OFF1) {
-                    up[i] = v1[i - S::OFF1];
-                } else {
-                    const float t = __shfl(v1[4 - S::OFF1 + i], lane - 16);
-                    up[i] = g > 0 ? t : 0.f;
+                for (int i = 0; i < 4; ++i)
+                    if (4 * g + i < S::P) stg[o.base + i * o.step] = v0[i] + up[i];
+                if (S::SETS2 && g == 3) {
+#pragma unroll
+                    for (int i = 0; i < S::OFF1; ++i) stg[o.base16 + i * o.step] = v1[4 - S::OFF1 + i];
                 }
+            };
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                put(acc0[r][0], acc1[r][0], rb + r, cc, cc < S::P);
+                put(acc0[r][1], acc1[r][1], rb + r, 8 + cc, 8 + cc < S::P);
             }
-            if (!ok) return;
-            const MfmaSlotCol o = mfma_slot_col<L>(g, b, cl, ri);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (4 * g + i < S::P) stg[o.base + i * o.step] = v0[i] + up[i];
-            if (S::SETS2 && g == 3) {
-#pragma unroll
-                for (int i = 0; i < S::OFF1; ++i) stg[o.base16 + i * o.step] = v1[4 - S::OFF1 + i];
+            for (int j = 0; j < NL; ++j) put(accL0[j], accL1[j], lb_b[j], lb_c[j], lb_ok[j]);
+            if (tid == 0) s_tk[0] = ticket;
+            __syncthreads();   // the image is whole and the ticket visible
+            const uint32_t cur_c = c;
+            c = cn;
+            ch = chn;
+            cn = s_tk[0];
+            // the next chunk's first batch loads during the copy-out, and the record
+            // of the chunk after it during the next chunk's steps
+            if (c < n && (uint32_t)nl < ch.count) {
+                nd = desc[ch.start + nl];
+                fv = fs[ch.start + nl];
             }
-        };
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            put(acc0[r][0], acc1[r][0], rb + r, cc, cc < S::P);
-            put(acc0[r][1], acc1[r][1], rb + r, 8 + cc, 8 + cc < S::P);
+            if (cn < n) chn = chunks[cn];
+            // the next chunk writes the image only after its first batch's barrier,
+            // which every wavefront reaches after its copy-out reads
+            float4 *dst = reinterpret_cast<float4 *>(slots + (int64_t)cur_c * L::SC);
+            const float4 *src = reinterpret_cast<const float4 *>(stg);
+            for (int i = tid; i < L::SC / 2; i += 256) stream_store(dst + i, src[i]);
         }
-#pragma unroll
-        for (int j = 0; j < NL; ++j) put(accL0[j], accL1[j], lb_b[j], lb_c[j], lb_ok[j]);
-        if (tid == 0) s_tk[0] = ticket;
-        __syncthreads();   // the image is whole and the ticket visible
-        const uint32_t cur_c = c;
-        c = cn;
-        ch = chn;
-        cn = s_tk[0];
-        // the next chunk's first batch loads during the copy-out, and the record
-        // of the chunk after it during the next chunk's steps
-        if (c < n && (uint32_t)nl < ch.count) {
-            nd = desc[ch.start + nl];
-            fv = fs[ch.start + nl];
-        }
-        if (cn < n) chn = chunks[cn];
-        // the next chunk writes the image only after its first batch's barrier,
-        // which every wavefront reaches after its copy-out reads
-        float4 *dst = reinterpret_cast<float4 *>(slots + (int64_t)cur_c * L::SC);
-        const float4 *src = reinterpret_cast<const float4 *>(stg);
-        for (int i = tid; i < L::SC / 2; i += 256) stream_store(dst + i, src[i]);
-    }
     };
 
     using std::integral_constant;

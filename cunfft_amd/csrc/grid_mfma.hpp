@@ -1,7 +1,7 @@
 // Adjoint tile accumulation on the matrix cores (FP32, d = 3, 8^3 tiles).
 //
-// Reference behaviour: the adjoint gridding loop of the reference's
-// spreading kernels (src/cunfft_kernel.cu, the adjoint convolution over the
+// Reference behaviour: the adjoint gridding of the reference's spreading
+// kernels (src/cunfft_kernel.cu:1098-1197, the adjoint convolution over the
 // 2m+2 window points per axis) sums f_j phi(x_j - cell) into every cell of
 // each node's window.  Here the sum over the nodes of one chunk is a matrix
 // product.  With the padded tile's cells as (plane a, row b, column c) and the
@@ -30,9 +30,9 @@
 //   weights per set row, row weights times the coefficient (re and im), column
 //   weights.  The nodes of a set are padded to a multiple of 4 with zero
 //   rows.  Descriptors and coefficients (the sorted copy, so no load waits
-//   on another) load one batch ahead.  Wavefront t < 3 evaluates axis t for the batch's
-//   nodes (lanes n and n + 32: node n, each half of the row's cells),
-//   wavefront 3 clears the padding rows.
+//   on another) load one batch ahead.  Wavefront t < 3 evaluates axis t for
+//   the batch's nodes (lanes n and n + 32: node n, each half of the row's
+//   cells), wavefront 3 clears the padding rows.
 // * The accumulators go to an LDS image of the chunk's slot (the fold's
 //   destination-blocked order), which the workgroup then copies out with
 //   16-byte stores: whole cache lines, where storing the accumulators

@@ -1,4 +1,5 @@
-// Adjoint tile accumulation on the matrix cores (FP32, d = 3, 8^3 tiles).
+// Adjoint tile accumulation on the matrix cores (FP32 and FP64, d = 3, 8^3
+// tiles).
 //
 // Reference behaviour: the adjoint gridding of the reference's spreading
 // kernels (src/cunfft_kernel.cu:1098-1197, the adjoint convolution over the
@@ -12,8 +13,11 @@
 //
 // A (planes x nodes) holding the plane weights and B (nodes x columns) the
 // products of coefficient, row and column weight, real and imaginary part in
-// adjacent columns.  v_mfma_f32_16x16x4f32 multiplies 16 planes by 4 nodes by
-// 16 columns, accumulating in FP32 (FP32 products, as the VALU form).
+// adjacent columns.  v_mfma_f32_16x16x4f32 (v_mfma_f64_16x16x4f64) multiplies
+// 16 planes by 4 nodes by 16 columns, accumulating in the plan's precision
+// (products in that precision, as the VALU form).  FP64 accumulators take
+// twice the registers, so an FP64 workgroup has 8 wavefronts sharing a tile's
+// rows where an FP32 one has 4.
 //
 // * Planes.  The padded tile has P = 8 + W - 1 planes (17 at W = 10) but one
 //   instruction covers 16.  A node's window spans W planes from its start
@@ -46,14 +50,39 @@
 namespace cunfft {
 
 typedef float mf4_t __attribute__((ext_vector_type(4)));
+typedef double md4_t __attribute__((ext_vector_type(4)));
 
-template <int W> struct MfmaSpread {
+// One 16x16x4 product per precision.  Both take A[m = lane % 16][k = lane / 16]
+// and B[k = lane / 16][n = lane % 16]; accumulator register i of lane l holds
+// row 4 (l / 16) + i (FP32) or l / 16 + 4 i (FP64) of column l % 16 (both
+// layouts checked on the GPU with an in-bounds probe).
+template <typename R> struct Mfma;
+template <> struct Mfma<float> {
+    using acc = mf4_t;
+    __device__ static acc op(float a, float b, const acc &c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __host__ __device__ static constexpr int row(int g, int i) { return 4 * g + i; }
+};
+template <> struct Mfma<double> {
+    using acc = md4_t;
+    __device__ static acc op(double a, double b, const acc &c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __host__ __device__ static constexpr int row(int g, int i) { return g + 4 * i; }
+};
+
+template <typename R, int W> struct MfmaSpread {
     static constexpr int T = 8;                         // tile extent per axis
     static constexpr int P = T + W - 1;                 // padded extent per axis
     static constexpr int OFF1 = P > 16 ? P - 16 : 0;    // first plane of the second set
     static constexpr bool SETS2 = OFF1 > 0;
     static constexpr int LO_MAX0 = 16 - W;              // set 0: window starts lo_0 <= LO_MAX0
-    static constexpr int NW = 4;                        // wavefronts per workgroup
+    // wavefronts per workgroup: FP64 accumulators take twice the registers, so
+    // twice the wavefronts share a tile's rows
+    static constexpr int NW = sizeof(R) == 8 ? 8 : 4;
     static constexpr int RW = (P + NW - 1) / NW;        // rows per wavefront (at most)
     static constexpr int NLO = P > 16 ? P - 16 : 0;     // columns past the two 8-column halves
     static constexpr int NLOB = (P * NLO * 2 + 15) / 16;   // their 16-column blocks
@@ -75,10 +104,10 @@ template <int W> struct MfmaSpread {
         for (int j = 0; j < LB; ++j) k += w + NW * j < NLOB ? 1 : 0;
         return k;
     }
-    // every wavefront's first row weight is 16-byte aligned in its table row
+    // (FP32) every wavefront's first row weight is 16-byte aligned in its table row
     static constexpr bool rows_aligned()
     {
-        bool ok = (O_B1 % 4 == 0) && (RS % 4 == 0) && (ROW % 4 == 0);
+        bool ok = sizeof(R) == 4 && (O_B1 % 4 == 0) && (RS % 4 == 0) && (ROW % 4 == 0);
         for (int w = 0; w < NW; ++w) ok = ok && row0(w) % 4 == 0;
         return ok;
     }
@@ -93,7 +122,19 @@ template <int W> struct MfmaSpread {
     static_assert(P <= RS, "a padded row fits the weight table stride");
 };
 
-template <int W> constexpr bool kMfmaSpread = W >= 4 && W <= 12;
+template <typename R, int W> constexpr bool kMfmaSpread = W >= 4 && W <= 12;
+// threads and resident workgroups per CU the kernel is built for
+template <typename R> constexpr int kMfmaThreads = sizeof(R) == 8 ? 512 : 256;
+template <typename R> constexpr int kMfmaPerCu = sizeof(R) == 8 ? 1 : 3;
+
+// Element offset (units of R) of part ri of cell (a, b, c) in a slot of the
+// 8^3 padded tile (the destination-blocked order of TileLayout::block_off).
+template <typename L> __device__ __forceinline__ int mfma_slot_off(int a, int b, int c, int ri)
+{
+    const int q1 = b >> 3, qL = c >> 3;
+    const int E1 = L::ext(L::T1, L::P1, q1), EL = L::ext(L::TL, L::PL, qL);
+    return 2 * (L::block_off(a >> 3, q1, qL) + ((a & 7) * E1 + (b & 7)) * EL + (c & 7)) + ri;
+}
 
 // Float offsets in a slot of the 8^3 padded tile (the destination-blocked
 // order of TileLayout::block_off) of part ri of the cells (a, b, c) a lane
@@ -126,25 +167,28 @@ template <typename L> __device__ __forceinline__ MfmaSlotCol mfma_slot_col(int g
 // instructions issue.
 // WIN: the window (inline, no calls: the accumulators stay in registers
 // through the weight evaluation).
-template <int W, int WIN>
-__global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<float> *__restrict__ desc,
-                                                          const cplx_t<float> *__restrict__ fs,
-                                                          const Chunk *__restrict__ chunks,
-                                                          uint32_t *__restrict__ n_chunks,
-                                                          cplx_t<float> *__restrict__ slots,
-                                                          WinConst<float> wc)
+template <typename R, int W, int WIN>
+__global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_mfma(
+    const NodeDesc<R> *__restrict__ desc, const cplx_t<R> *__restrict__ fs, const Chunk *__restrict__ chunks,
+    uint32_t *__restrict__ n_chunks, cplx_t<R> *__restrict__ slots, WinConst<R> wc)
 {
-    using S = MfmaSpread<W>;
-    using L = TileLayout<float, 3, W, 4, 0>;   // slot layout (shared with the fold)
+    using S = MfmaSpread<R, W>;
+    using M = Mfma<R>;
+    using acc_t = typename M::acc;
+    using V16 = vec16_t<R>;                 // 16 bytes of R
+    constexpr int E16 = 16 / (int)sizeof(R);
+    constexpr int NT = S::NW * 64;
+    static_assert(NT == kMfmaThreads<R>, "one thread per lane of the workgroup's wavefronts");
+    using L = TileLayout<R, 3, W, 4, 0>;    // slot layout (shared with the fold)
     static_assert(L::T0 == S::T && L::T1 == S::T && L::TL == S::T && L::P0 == S::P, "8^3 tiles");
     constexpr int LBN = S::LB > 0 ? S::LB : 1;
-    __shared__ __align__(16) float tw[S::NPOS][S::ROW];
-    __shared__ __align__(16) float stg[2 * L::SC];   // the slot image
+    __shared__ __align__(16) R tw[S::NPOS][S::ROW];
+    __shared__ __align__(16) R stg[2 * L::SC];   // the slot image
     __shared__ uint32_t s_tk[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // operand lanes: A[m = lane % 16][k = lane / 16], B[k = lane / 16][n = lane % 16],
-    // accumulator register i: D[4 (lane / 16) + i][lane % 16]
+    // accumulator register i: D[M::row(lane / 16, i)][lane % 16]
     const int g = lane >> 4, nn = lane & 15, ri = nn & 1, cc = nn >> 1;
     const int rb = wave * S::P / S::NW;
     const int nrows = (wave + 1) * S::P / S::NW - rb;
@@ -172,8 +216,8 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
     // wavefront share the weight evaluation of a node)
     static_assert(S::NB == 32, "a batch fills half a wavefront");
     const int nl = lane & (S::NB - 1), half = lane >> 5;
-    NodeDesc<float> nd{};
-    cplx_t<float> fv = cmake<float>(0.f, 0.f);
+    NodeDesc<R> nd{};
+    cplx_t<R> fv = cmake<R>((R)0, (R)0);
     Chunk ch{0u, 0u, 0u, 0u}, chn{0u, 0u, 0u, 0u};
     if (cn < n) chn = chunks[cn];
     if (c < n) {
@@ -192,9 +236,9 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
         ox1[j] = S::O_B1 + ri * S::RS + lb_b[j];
         ox2[j] = S::O_B2 + lb_c[j];
     }
-    const float *rowg = &tw[0][0] + g * S::ROW;   // step ks reads row 4 ks + g
-    const float *rowb = rowg + ob1 + rb;          // the wavefront's first row weight
-    const mf4_t z4 = {0.f, 0.f, 0.f, 0.f};
+    const R *rowg = &tw[0][0] + g * S::ROW;   // step ks reads row 4 ks + g
+    const R *rowb = rowg + ob1 + rb;          // the wavefront's first row weight
+    const acc_t z4 = {(R)0, (R)0, (R)0, (R)0};
 
     // The chunk loop of a wavefront with NR rows and NL extra blocks, both
     // fixed at compile time (one instance per wavefront shape, at most two per
@@ -202,19 +246,19 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
     // 4-node step, and the accumulators of one instance only.
     auto body = [&](auto nr_c, auto nl_c) {
         constexpr int NR = decltype(nr_c)::value, NL = decltype(nl_c)::value;
-        mf4_t acc0[S::RW][2], acc1[S::RW][2], accL0[LBN], accL1[LBN];
+        acc_t acc0[S::RW][2], acc1[S::RW][2], accL0[LBN], accL1[LBN];
 
         // The 4-node steps [0, ks0) into set 0 and [ks0, ks1) into set 1; the
         // next step's operands load while the current step's MFMAs issue (two
         // operand sets in turn, so no copies between steps).
         auto ksteps = [&](int ks0, int ks1) {
             struct Ops {
-                float a, w2lo, w2hi, w1[NR], x1[NL > 0 ? NL : 1], x2[NL > 0 ? NL : 1];
+                R a, w2lo, w2hi, w1[NR], x1[NL > 0 ? NL : 1], x2[NL > 0 ? NL : 1];
             };
             // per-lane table pointers of a step's operands; step ks + 1 is 4 rows on
             constexpr int STEP = 4 * S::ROW;
             struct Ptrs {
-                const float *a, *w2, *w1, *x1[NL > 0 ? NL : 1], *x2[NL > 0 ? NL : 1];
+                const R *a, *w2, *w1, *x1[NL > 0 ? NL : 1], *x2[NL > 0 ? NL : 1];
             };
             auto ptrs = [&](int ks) {
                 Ptrs q;
@@ -244,7 +288,7 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 o.a = q.a[d * STEP];
                 o.w2lo = q.w2[d * STEP];
                 o.w2hi = q.w2[d * STEP + 8];
-                const float *w1p = q.w1 + d * STEP;
+                const R *w1p = q.w1 + d * STEP;
                 if constexpr (S::rows_aligned() && NR >= 4) {
                     // the first four row weights in one 16-byte read
                     const float4 q = *reinterpret_cast<const float4 *>(w1p);
@@ -265,19 +309,19 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 }
                 return o;
             };
-            auto mma = [&](const Ops &o, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+            auto mma = [&](const Ops &o, acc_t(&acc)[S::RW][2], acc_t(&accL)[LBN]) {
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2lo, acc[r][0], 0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, o.w1[r] * o.w2hi, acc[r][1], 0, 0, 0);
+                    acc[r][0] = M::op(o.a, o.w1[r] * o.w2lo, acc[r][0]);
+                    acc[r][1] = M::op(o.a, o.w1[r] * o.w2hi, acc[r][1]);
                 }
 #pragma unroll
                 for (int j = 0; j < NL; ++j) {
-                    const float bv = lb_ok[j] ? o.x1[j] * o.x2[j] : 0.f;
-                    accL[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a, bv, accL[j], 0, 0, 0);
+                    const R bv = lb_ok[j] ? o.x1[j] * o.x2[j] : (R)0;
+                    accL[j] = M::op(o.a, bv, accL[j]);
                 }
             };
-            auto run = [&](int k0, int k1, mf4_t(&acc)[S::RW][2], mf4_t(&accL)[LBN]) {
+            auto run = [&](int k0, int k1, acc_t(&acc)[S::RW][2], acc_t(&accL)[LBN]) {
                 if (k0 >= k1) return;
                 // (the scheduling barriers keep each step's loads ahead of the
                 // previous step's instructions, which the scheduler would sink
@@ -318,8 +362,8 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 // the batch's node, loaded one batch ahead: opaque from here on, so
                 // no use of it is moved up to its load (which would wait on the
                 // load there, before the previous batch's steps)
-                uint32_t loc = nd.loc;
-                float nf0 = nd.f[0], nf1 = nd.f[1], nf2 = nd.f[2], fre = fv.x, fim = fv.y;
+                uint32_t loc = (uint32_t)nd.loc;
+                R nf0 = nd.f[0], nf1 = nd.f[1], nf2 = nd.f[2], fre = fv.x, fim = fv.y;
                 asm volatile("" : "+v"(loc), "+v"(nf0), "+v"(nf1), "+v"(nf2), "+v"(fre), "+v"(fim));
                 const int lo0 = loc & 255, lo1 = (loc >> 8) & 255, loL = (loc >> 16) & 255;
                 const bool s1 = S::SETS2 && valid && lo0 > S::LO_MAX0;
@@ -333,44 +377,45 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 if (wave < 3) {
                     if (valid) {
                         const int t = wave;   // axis
-                        const float ft = t == 0 ? nf0 : (t == 1 ? nf1 : nf2);
-                        const float kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
-                        const float nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
-                        const float shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
+                        const R ft = t == 0 ? nf0 : (t == 1 ? nf1 : nf2);
+                        const R kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
+                        const R nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
+                        const R shpt = t == 0 ? wc.shp[0] : (t == 1 ? wc.shp[1] : wc.shp[2]);
                         // cell x of the table row has tap l = x - lo (weight zero outside [0, W))
                         const int lo = t == 0 ? lo0 - (s1 ? S::OFF1 : 0) : (t == 1 ? lo1 : loL);
                         const int nx = t == 0 ? 16 : S::RS;
-                        float *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
-                        const float sc0 = t == 1 ? fre : 1.f, sc1 = fim;
+                        R *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
+                        const R sc0 = t == 1 ? fre : (R)1, sc1 = fim;
                         // half h of the wavefront writes the quads x0 = 4h, 4h + 8, ...
 #pragma unroll 1
                         for (int x0 = 4 * half; x0 < nx; x0 += 8) {
-                            float v[4];
+                            R v[4];
 #pragma unroll
                             for (int u = 0; u < 4; ++u) {
                                 const int l = x0 + u - lo;
-                                const float sv = ft + (float)(wc.m - l);
-                                float w;
-                                if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<float>(sv, (float)wc.m, shpt);
-                                else if constexpr (WIN == WIN_SINC) w = sinc_phi<float>(sv, wc.m, shpt);
-                                else w = nrt * wexp<float>(-kbt * sv * sv);
+                                const R sv = ft + (R)(wc.m - l);
+                                R w;
+                                if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<R>(sv, (R)wc.m, shpt);
+                                else if constexpr (WIN == WIN_SINC) w = sinc_phi<R>(sv, wc.m, shpt);
+                                else w = nrt * wexp<R>(-kbt * sv * sv);
                                 v[u] = l >= 0 && l < W ? w : 0.f;
                             }
-                            *reinterpret_cast<float4 *>(row + x0) =
-                                make_float4(v[0] * sc0, v[1] * sc0, v[2] * sc0, v[3] * sc0);
-                            if (t == 1)
-                                *reinterpret_cast<float4 *>(row + S::RS + x0) =
-                                    make_float4(v[0] * sc1, v[1] * sc1, v[2] * sc1, v[3] * sc1);
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) row[x0 + u] = v[u] * sc0;
+                            if (t == 1) {
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) row[S::RS + x0 + u] = v[u] * sc1;
+                            }
                         }
                     }
-                } else {
+                } else if (wave == 3) {
                     // zero rows for the padding positions [n0, p0pad) and [p0pad + n1, npos)
-                    constexpr int U = S::ROW / 4;   // 16-byte units per row
+                    constexpr int U = S::ROW / E16;   // 16-byte units per row
                     const int pad0 = p0pad - n0, npad = npos - n0 - n1;
                     for (int i = lane; i < npad * U; i += 64) {
                         const int pi = i / U, u = i - pi * U;
                         const int pp = pi < pad0 ? n0 + pi : p0pad + n1 + (pi - pad0);
-                        reinterpret_cast<float4 *>(&tw[pp][0])[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        reinterpret_cast<V16 *>(&tw[pp][0])[u] = V16{};
                     }
                 }
                 __syncthreads();
@@ -384,29 +429,50 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
                 }
                 ksteps(p0pad >> 2, S::SETS2 ? npos >> 2 : p0pad >> 2);
             }
-            // the sum of the two sets per plane into the slot image; lane group g holds
-            // planes 4g + i of set 0 and 4g + i + OFF1 of set 1
-            auto put = [&](const mf4_t &v0, const mf4_t &v1, int b, int cl, bool ok) {
-                float up[4];   // set 1 at plane 4g + i
+            // the sum of the two sets per plane into the slot image; register i of lane
+            // group g holds plane M::row(g, i) of set 0 and M::row(g, i) + OFF1 of set 1
+            auto put = [&](const acc_t &v0, const acc_t &v1, int b, int cl, bool ok) {
+                R up[4];   // set 1 at plane M::row(g, i)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     if (!S::SETS2) {
-                        up[i] = 0.f;
-                    } else if (i >= S::OFF1) {
-                        up[i] = v1[i - S::OFF1];
+                        up[i] = (R)0;
+                    } else if constexpr (sizeof(R) == 4) {
+                        // plane 4g + i of set 1 is row 4g + i - OFF1: this lane
+                        // group's register i - OFF1, or the previous group's
+                        if (i >= S::OFF1) {
+                            up[i] = v1[i - S::OFF1];
+                        } else {
+                            const R t = __shfl(v1[4 - S::OFF1 + i], lane - 16);
+                            up[i] = g > 0 ? t : (R)0;
+                        }
                     } else {
-                        const float t = __shfl(v1[4 - S::OFF1 + i], lane - 16);
-                        up[i] = g > 0 ? t : 0.f;
+                        // plane g + 4i of set 1 is row g - OFF1 + 4i: register i of lane
+                        // group g - OFF1, or register i - 1 of group g - OFF1 + 4
+                        const R t1 = __shfl(v1[i], lane - 16 * S::OFF1);
+                        const R t2 = i > 0 ? __shfl(v1[i > 0 ? i - 1 : 0], lane + 16 * (4 - S::OFF1)) : (R)0;
+                        up[i] = g >= S::OFF1 ? t1 : t2;
                     }
                 }
                 if (!ok) return;
-                const MfmaSlotCol o = mfma_slot_col<L>(g, b, cl, ri);
+                if constexpr (sizeof(R) == 4) {
+                    const MfmaSlotCol o = mfma_slot_col<L>(g, b, cl, ri);
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (4 * g + i < S::P) stg[o.base + i * o.step] = v0[i] + up[i];
-                if (S::SETS2 && g == 3) {
+                    for (int i = 0; i < 4; ++i)
+                        if (4 * g + i < S::P) stg[o.base + i * o.step] = v0[i] + up[i];
+                    if (S::SETS2 && g == 3) {
 #pragma unroll
-                    for (int i = 0; i < S::OFF1; ++i) stg[o.base16 + i * o.step] = v1[4 - S::OFF1 + i];
+                        for (int i = 0; i < S::OFF1; ++i) stg[o.base16 + i * o.step] = v1[4 - S::OFF1 + i];
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int a = g + 4 * i;
+                        if (a < S::P) stg[mfma_slot_off<L>(a, b, cl, ri)] = v0[i] + up[i];
+                    }
+                    // planes 16 + j (rows 16 - OFF1 + j of set 1: register 3 of group 4 - OFF1 + j)
+                    if (S::SETS2 && g >= 4 - S::OFF1)
+                        stg[mfma_slot_off<L>(16 + g - (4 - S::OFF1), b, cl, ri)] = v1[3];
                 }
             };
 #pragma unroll
@@ -431,9 +497,9 @@ __global__ __launch_bounds__(256, 3) void k_tile_spread_mfma(const NodeDesc<floa
             if (cn < n) chn = chunks[cn];
             // the next chunk writes the image only after its first batch's barrier,
             // which every wavefront reaches after its copy-out reads
-            float4 *dst = reinterpret_cast<float4 *>(slots + (int64_t)cur_c * L::SC);
-            const float4 *src = reinterpret_cast<const float4 *>(stg);
-            for (int i = tid; i < L::SC / 2; i += 256) stream_store(dst + i, src[i]);
+            V16 *dst = reinterpret_cast<V16 *>(slots + (int64_t)cur_c * L::SC);
+            const V16 *src = reinterpret_cast<const V16 *>(stg);
+            for (int i = tid; i < 2 * L::SC / E16; i += NT) stream_store(dst + i, src[i]);
         }
     };
 

@@ -24,9 +24,9 @@
 // Adjoint: the accumulated padded tile of each chunk is written to a slot in
 // a destination-blocked order, and k_tile_fold sums, for every grid tile, the
 // slot blocks of the chunks whose padded tiles cover it: no atomics, no zero
-// fill, and a result that does not depend on scheduling.  The FP32 3D
-// adjoint on 8^3 tiles accumulates its slots on the matrix cores instead
-// (grid_mfma.hpp); the fold is shared.
+// fill, and a result that does not depend on scheduling.  The 3D adjoint
+// on 8^3 tiles (FP32 and FP64) accumulates its slots on the matrix cores
+// instead (grid_mfma.hpp); the fold is shared.
 #pragma once
 #include <map>
 #include <mutex>
@@ -2075,8 +2075,8 @@ template <typename T> struct SpreadOp {
             cplx_t<T> *slots = static_cast<cplx_t<T> *>(b.slots);
             cplx_t<T> *fs = static_cast<cplx_t<T> *>(b.fs);
             using L = TileLayout<T, D, W, NWS, V>;
-            // the FP32 3D spreads read f through the permutation themselves
-            constexpr bool MFMA = sizeof(T) == 4 && D == 3 && V == 0 && kMfmaSpread<W>;
+            // the FP32 3D VALU spreads read f through the permutation themselves
+            constexpr bool MFMA = D == 3 && V == 0 && kMfmaSpread<T, W>;
             constexpr bool QUEUE = D == 3 && L::G == 1;
             constexpr bool PERM_F = QUEUE && !MFMA && sizeof(T) == 4;
             if (!PERM_F || b.compensated)
@@ -2100,14 +2100,14 @@ template <typename T> struct SpreadOp {
             if constexpr (MFMA) {
                 // persistent: clear the work counter (a kernel, not a memset), then one
                 // resident grid
-                auto km = ge.win == WIN_KAISER_BESSEL ? k_tile_spread_mfma<W, WIN_KAISER_BESSEL>
-                          : ge.win == WIN_SINC        ? k_tile_spread_mfma<W, WIN_SINC>
-                                                      : k_tile_spread_mfma<W, WIN_GAUSSIAN>;
+                auto km = ge.win == WIN_KAISER_BESSEL ? k_tile_spread_mfma<T, W, WIN_KAISER_BESSEL>
+                          : ge.win == WIN_SINC        ? k_tile_spread_mfma<T, W, WIN_SINC>
+                                                      : k_tile_spread_mfma<T, W, WIN_GAUSSIAN>;
+                constexpr int NT = MfmaSpread<T, W>::NW * 64;
                 k_clear_u32<<<1, 64, 0, s>>>(b.n_chunks + 1);
-                const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(km, 256));
-                km<<<grid, 256, 0, s>>>(reinterpret_cast<const NodeDesc<float> *>(b.desc),
-                                        reinterpret_cast<const cplx_t<float> *>(fs), b.chunks, b.n_chunks,
-                                        reinterpret_cast<cplx_t<float> *>(slots), win_const<float>(ge));
+                const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(km, NT));
+                km<<<grid, NT, 0, s>>>(static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
+                                       win_const<T>(ge));
             } else if constexpr (QUEUE) {
                 // persistent held-unit spread: clear the work counter (a kernel, not a
                 // memset, so the launch sequence can be captured in a graph), then one

@@ -1,6 +1,6 @@
-"""The FP32 3D adjoint on the matrix cores (k_tile_spread_mfma, 8^3 tiles,
-configuration 0) against the oracle's NFFT adjoint on FP32-rounded inputs,
-to 1e-5 relative l2:
+"""The 3D adjoint on the matrix cores (k_tile_spread_mfma, 8^3 tiles,
+configuration 0), FP32 and FP64, against the oracle's NFFT adjoint (FP32: on
+FP32-rounded inputs, to 1e-5 relative l2; FP64: to 1e-12):
 
 * every window width it takes (m = 1..5, W = 4..12; m = 1..3 use one 16-plane
   set, m = 4, 5 two), uniform nodes;
@@ -14,7 +14,7 @@ to 1e-5 relative l2:
 * a node count that leaves partial batches and odd set sizes (the zero
   padding rows of each set);
 * the same chunk sums as the VALU spread of configuration 1 (8x8x16 tiles):
-  both within 1e-5 of the oracle and of each other;
+  both within the tolerance of the oracle and of each other;
 * a given binning gives a bit-identical grid call to call.
 """
 import numpy as np
@@ -22,17 +22,25 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+TOL = {"single": 1e-5, "double": 1e-12}
+
+
+@pytest.fixture(params=["single", "double"])
+def prec(request):
+    return request.param
+
 
 def crandn(r, n):
     return r.standard_normal(n) + 1j * r.standard_normal(n)
 
 
-def q32(a):
-    return a.astype(np.complex64).astype(np.complex128)
+def rounded(a, prec):
+    """The coefficients as the plan holds them (FP32-rounded for single)."""
+    return a.astype(np.complex64).astype(np.complex128) if prec == "single" else a
 
 
-def adjoint(gpu, N, n, m, x, fv, flags=0, config=0, reps=1):
-    with gpu.Plan(N, x.shape[0], n=n, m=m, precision="single",
+def adjoint(gpu, prec, N, n, m, x, fv, flags=0, config=0, reps=1):
+    with gpu.Plan(N, x.shape[0], n=n, m=m, precision=prec,
                   flags=gpu.CUFFT_INIT | gpu.CUFFT_BOTH | flags) as p:
         p.config = config
         p.x[:] = x
@@ -47,29 +55,29 @@ def adjoint(gpu, N, n, m, x, fv, flags=0, config=0, reps=1):
 
 
 @pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
-def test_every_window_width_matches_the_oracle(gpu, orc, m):
+def test_every_window_width_matches_the_oracle(gpu, orc, prec, m):
     N, n, M = [32, 32, 32], [64, 64, 64], 20000
     r = np.random.default_rng(500 + m)
     x = r.uniform(-0.5, 0.5, (M, 3))
     fv = crandn(r, M)
-    h = adjoint(gpu, N, n, m, x, fv)
-    err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, q32(fv)), h)
-    print(f"m={m} W={2 * m + 2} rel-l2 vs oracle {err:.3e}")
-    assert err <= 1e-5
+    h = adjoint(gpu, prec, N, n, m, x, fv)
+    err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, rounded(fv, prec)), h)
+    print(f"{prec} m={m} W={2 * m + 2} rel-l2 vs oracle {err:.3e}")
+    assert err <= TOL[prec]
 
 
 @pytest.mark.parametrize("kind,flag", [("kaiser_bessel", "KAISER_BESSEL"), ("sinc", "SINC")])
-def test_other_windows_match_the_oracle(gpu, orc, kind, flag):
+def test_other_windows_match_the_oracle(gpu, orc, prec, kind, flag):
     N, n, m, M = [32, 32, 32], [64, 64, 64], 4, 20000
     r = np.random.default_rng(520)
     x = r.uniform(-0.5, 0.5, (M, 3))
     fv = crandn(r, M)
-    h = adjoint(gpu, N, n, m, x, fv, flags=getattr(gpu, flag))
+    h = adjoint(gpu, prec, N, n, m, x, fv, flags=getattr(gpu, flag))
     with orc.window(kind):
-        ref = orc.nfft_adjoint(N, n, m, x, q32(fv))
+        ref = orc.nfft_adjoint(N, n, m, x, rounded(fv, prec))
     err = orc.rel_l2(ref, h)
-    print(f"{kind} rel-l2 vs oracle {err:.3e}")
-    assert err <= 1e-5
+    print(f"{prec} {kind} rel-l2 vs oracle {err:.3e}")
+    assert err <= TOL[prec]
 
 
 def starts_at_plane7(r, M, n, m):
@@ -79,20 +87,20 @@ def starts_at_plane7(r, M, n, m):
     return c / n
 
 
-def test_second_plane_set_alone(gpu, orc):
+def test_second_plane_set_alone(gpu, orc, prec):
     N, n, m, M = [32, 32, 32], [64, 64, 64], 4, 12000
     r = np.random.default_rng(530)
     x = r.uniform(-0.5, 0.5, (M, 3))
     x[:, 0] = starts_at_plane7(r, M, n[0], m)
     assert np.all([(orc.window_start(n[0], v, m) % n[0]) % 8 == 7 for v in x[:200, 0]])
     fv = crandn(r, M)
-    h = adjoint(gpu, N, n, m, x, fv)
-    err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, q32(fv)), h)
-    print(f"all windows from plane 7: rel-l2 {err:.3e}")
-    assert err <= 1e-5
+    h = adjoint(gpu, prec, N, n, m, x, fv)
+    err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, rounded(fv, prec)), h)
+    print(f"{prec} all windows from plane 7: rel-l2 {err:.3e}")
+    assert err <= TOL[prec]
 
 
-def test_dense_tile_and_partial_batches(gpu, orc):
+def test_dense_tile_and_partial_batches(gpu, orc, prec):
     N, n, m = [32, 32, 32], [64, 64, 64], 4
     r = np.random.default_rng(540)
     # 6000 nodes whose windows start in one 8^3 grid tile (rint(n x) - m in [8, 16) on
@@ -100,22 +108,23 @@ def test_dense_tile_and_partial_batches(gpu, orc):
     dense = (8 + m + r.uniform(-0.49, 7.49, (6000, 3))) / n[0]
     x = np.concatenate([dense, r.uniform(-0.5, 0.5, (4321, 3))])
     fv = crandn(r, x.shape[0])
-    h = adjoint(gpu, N, n, m, x, fv)
-    err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, q32(fv)), h)
-    print(f"dense tile + 4321 uniform: rel-l2 {err:.3e}")
-    assert err <= 1e-5
+    h = adjoint(gpu, prec, N, n, m, x, fv)
+    err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, rounded(fv, prec)), h)
+    print(f"{prec} dense tile + 4321 uniform: rel-l2 {err:.3e}")
+    assert err <= TOL[prec]
 
 
-def test_agrees_with_the_valu_spread_and_repeats_bit_for_bit(gpu, orc):
+def test_agrees_with_the_valu_spread_and_repeats_bit_for_bit(gpu, orc, prec):
     N, n, m, M = [32, 32, 32], [64, 64, 64], 4, 30000
     r = np.random.default_rng(550)
     c = r.normal(0.0, 0.08, (M, 3))
     x = np.concatenate([c - np.floor(c + 0.5), r.uniform(-0.5, 0.5, (M, 3))])
     fv = crandn(r, x.shape[0])
-    ha, hb = adjoint(gpu, N, n, m, x, fv, config=0, reps=2)
-    hv = adjoint(gpu, N, n, m, x, fv, config=1)
-    ref = orc.nfft_adjoint(N, n, m, x, q32(fv))
+    ha, hb = adjoint(gpu, prec, N, n, m, x, fv, config=0, reps=2)
+    hv = adjoint(gpu, prec, N, n, m, x, fv, config=1)
+    ref = orc.nfft_adjoint(N, n, m, x, rounded(fv, prec))
     ea, ev, eav = orc.rel_l2(ref, ha), orc.rel_l2(ref, hv), orc.rel_l2(hv, ha)
-    print(f"matrix cores {ea:.3e}, VALU spread {ev:.3e}, between them {eav:.3e}")
-    assert ea <= 1e-5 and ev <= 1e-5 and eav <= 1e-5
+    print(f"{prec} matrix cores {ea:.3e}, VALU spread {ev:.3e}, between them {eav:.3e}")
+    tol = TOL[prec]
+    assert ea <= tol and ev <= tol and eav <= tol
     assert np.array_equal(ha, hb)

@@ -36,6 +36,7 @@ M_NODES = 2 ** 21
 CUT = 4
 HBM_PEAK_GBS = 8000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
+MFMA_FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 matrix-core peak
 
 
 def log(*a):
@@ -549,7 +550,7 @@ def main():
     shapes = ["8x8x8", "8x8x16", "16x16x8"]
     names = {"fwd_gather_ms": f"k_tile_gather<float,3,10,NW,{cf}> ({shapes[cf]} tiles, forward "
                               "gridding)" if cf < 3 else "k_gather_tiled<float,3,10> (node lanes)",
-             "adj_scatter_ms": "k_tile_spread_mfma<10,0> (8x8x8 tiles, adjoint tile accumulation on "
+             "adj_scatter_ms": "k_tile_spread_mfma<float,10,0> (8x8x8 tiles, adjoint tile accumulation on "
                                "the matrix cores)" if ca == 0
                                else f"k_tile_spread_queue<float,3,10,4,{ca}> ({shapes[ca]} tiles, "
                                "adjoint tile accumulation)"
@@ -565,14 +566,21 @@ def main():
                               "(profiles/traffic_gridding_3d128_fp32.txt); null when that "
                               "summary does not cover the configuration the autotuner chose",
             "algorithmic_bytes": grid_bytes(8), "avg_launch_ms": t32[kern]}
+    # the spread's products on the matrix cores: the useful flops (two FMAs per window
+    # cell and node, complex coefficient) against the dense MFMA peak of the precision
+    useful = M_NODES * (2 * CUT + 2) ** 3 * 4
+
+    def matrix_cores(ms, peak):
+        return {"useful_flops": useful, "useful_tflops": useful / (ms * 1e-3) / 1e12,
+                "peak_tflops": peak, "frac": useful / (ms * 1e-3) / 1e12 / peak}
+
     if kern == "adj_scatter_ms" and ca == 0:
-        # the spread's products on the matrix cores: the useful FP32 flops (two FMAs per
-        # window cell and node) against the dense FP32 MFMA peak
-        useful = M_NODES * (2 * CUT + 2) ** 3 * 4
-        roof["matrix_cores"] = {"useful_flops": useful,
-                                "useful_tflops": useful / (t32[kern] * 1e-3) / 1e12,
-                                "peak_tflops": MFMA_FP32_PEAK_TFLOPS,
-                                "frac": useful / (t32[kern] * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS}
+        roof["matrix_cores"] = matrix_cores(t32[kern], MFMA_FP32_PEAK_TFLOPS)
+    fp64 = {"value": value64, "ms_per_step": t64["ms_per_step"], "fp32_speedup_over_fp64": value / value64}
+    if t64["config"][1] == 0:
+        fp64["adj_kernel"] = ("k_tile_spread_mfma<double,10,0> (8x8x8 tiles, adjoint tile accumulation "
+                              "on the FP64 matrix cores)")
+        fp64["adj_matrix_cores"] = matrix_cores(t64["adj_scatter_ms"], MFMA_FP64_PEAK_TFLOPS)
     cpu = None
     if not a.no_cpu_baseline:
         log("bench: CPU baseline")
@@ -585,8 +593,7 @@ def main():
         "data": "synthetic: half uniform, half N(0,0.1^2) wrapped nodes; N(0,1) coefficients",
         "config": {"workload": "3D NFFT forward+adjoint, N=128^3, n=256^3, m=4, M=2^21 nodes",
                    "N": N3, "n": NN3, "m": CUT, "M": M_NODES, "precision": "fp32"},
-        "fp64": {"value": value64, "ms_per_step": t64["ms_per_step"],
-                 "fp32_speedup_over_fp64": value / value64},
+        "fp64": fp64,
         "steps_fp32_ms": t32, "steps_fp64_ms": t64, "accuracy": acc,
         "roofline": roof, "cpu_baseline": cpu, "batched_2d": batch, "pcie_inclusive": pcie,
     }

@@ -236,6 +236,24 @@ __global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_
         ox1[j] = S::O_B1 + ri * S::RS + lb_b[j];
         ox2[j] = S::O_B2 + lb_c[j];
     }
+    // The weight prologue: PW wavefronts per axis (two where the workgroup has
+    // six or more), wavefront 3 PW clears the padding rows.  Wavefront wave < 3 PW
+    // evaluates axis wave / PW; its lanes write the quads x0 = 4 q + 8 PW k of a
+    // row, q = PW half + wave % PW.
+    constexpr int PW = S::NW >= 6 ? 2 : 1;
+    // FP64 Gaussian: a quad's weights from two exps,
+    // exp(-b (s0 - u)^2) = exp(-b s0^2) exp(2 b s0)^u exp(-b u^2), with the last
+    // factor per axis (s0 is at most m + 4 in magnitude where a weight is used,
+    // so neither factor leaves the FP64 range)
+    constexpr bool GQUAD = WIN == WIN_GAUSSIAN && sizeof(R) == 8;
+    R gq1 = (R)1, gq2 = (R)1, gq3 = (R)1;
+    if constexpr (GQUAD) {
+        const int ax = wave / PW < 3 ? wave / PW : 2;
+        const R kb_ax = ax == 0 ? wc.kb[0] : (ax == 1 ? wc.kb[1] : wc.kb[2]);
+        gq1 = wexp<R>(-kb_ax);
+        gq2 = wexp<R>((R)-4 * kb_ax);
+        gq3 = wexp<R>((R)-9 * kb_ax);
+    }
     const R *rowg = &tw[0][0] + g * S::ROW;   // step ks reads row 4 ks + g
     const R *rowb = rowg + ob1 + rb;          // the wavefront's first row weight
     const acc_t z4 = {(R)0, (R)0, (R)0, (R)0};
@@ -374,9 +392,10 @@ __global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_
                 const uint64_t mine = s1 ? m1 : m0;
                 const int rank = __popcll(mine & ((1ull << nl) - 1ull));
                 const int pos = s1 ? p0pad + rank : rank;
-                if (wave < 3) {
+                if (wave < 3 * PW) {
                     if (valid) {
-                        const int t = wave;   // axis
+                        const int t = PW == 1 ? wave : wave / PW;   // axis
+                        const int tq = PW == 1 ? half : PW * half + (wave - t * PW);
                         const R ft = t == 0 ? nf0 : (t == 1 ? nf1 : nf2);
                         const R kbt = t == 0 ? wc.kb[0] : (t == 1 ? wc.kb[1] : wc.kb[2]);
                         const R nrt = t == 0 ? wc.nrm[0] : (t == 1 ? wc.nrm[1] : wc.nrm[2]);
@@ -386,19 +405,29 @@ __global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_
                         const int nx = t == 0 ? 16 : S::RS;
                         R *row = &tw[pos][0] + (t == 0 ? 0 : (t == 1 ? S::O_B1 : S::O_B2));
                         const R sc0 = t == 1 ? fre : (R)1, sc1 = fim;
-                        // half h of the wavefront writes the quads x0 = 4h, 4h + 8, ...
 #pragma unroll 1
-                        for (int x0 = 4 * half; x0 < nx; x0 += 8) {
+                        for (int x0 = 4 * tq; x0 < nx; x0 += 8 * PW) {
                             R v[4];
+                            if constexpr (GQUAD) {
+                                const R s0 = ft + (R)(wc.m - (x0 - lo));
+                                const R e0 = nrt * wexp<R>(-kbt * s0 * s0);
+                                const R r = wexp<R>((R)2 * kbt * s0);
+                                const R r2 = r * r;
+                                v[0] = e0;
+                                v[1] = e0 * r * gq1;
+                                v[2] = e0 * r2 * gq2;
+                                v[3] = e0 * (r2 * r) * gq3;
+                            }
 #pragma unroll
                             for (int u = 0; u < 4; ++u) {
                                 const int l = x0 + u - lo;
                                 const R sv = ft + (R)(wc.m - l);
                                 R w;
-                                if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<R>(sv, (R)wc.m, shpt);
+                                if constexpr (GQUAD) w = v[u];
+                                else if constexpr (WIN == WIN_KAISER_BESSEL) w = kb_phi<R>(sv, (R)wc.m, shpt);
                                 else if constexpr (WIN == WIN_SINC) w = sinc_phi<R>(sv, wc.m, shpt);
                                 else w = nrt * wexp<R>(-kbt * sv * sv);
-                                v[u] = l >= 0 && l < W ? w : 0.f;
+                                v[u] = l >= 0 && l < W ? w : (R)0;
                             }
 #pragma unroll
                             for (int u = 0; u < 4; ++u) row[x0 + u] = v[u] * sc0;
@@ -408,7 +437,7 @@ __global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_
                             }
                         }
                     }
-                } else if (wave == 3) {
+                } else if (wave == 3 * PW) {
                     // zero rows for the padding positions [n0, p0pad) and [p0pad + n1, npos)
                     constexpr int U = S::ROW / E16;   // 16-byte units per row
                     const int pad0 = p0pad - n0, npad = npos - n0 - n1;

@@ -548,4 +548,132 @@ __global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_
             return body(integral_constant<int, RLO + 1>{}, integral_constant<int, LLO + 1>{});
 }
 
+// ------------------------------------------------- 2D, 16^2 tiles ----
+//
+// The 2D adjoint on 16^2 tiles (FP32): tile[a][c] = sum_j wa_j[a] (f_j wc_j[c]),
+// a rank-K product of A (rows x nodes) and B (nodes x columns, re/im in
+// adjacent columns).  The padded tile has P = 16 + W - 1 <= 32 rows, so one
+// v_mfma_f32_32x32x2f32 covers every row: A[m = lane % 32][k = lane / 32],
+// B[k = lane / 32][n = lane % 32], and accumulator register i of lane l holds
+// D[i % 4 + 8 (i / 4) + 4 (l / 32)][l % 32] (checked on the GPU with an
+// in-bounds probe).  Two 32-column blocks hold cells [0, 16) and [16, P).
+// One wavefront per chunk; per batch of 32 nodes the weights go to LDS in
+// padded-tile coordinates (zero outside the window), lanes n and n + 32
+// evaluating node n's row and column weights; then two MFMAs per two nodes.
+// The accumulators go straight to the slot: for a fixed register the 32
+// lanes of a half-wavefront store 32 consecutive floats of one row.
+typedef float mf16_t __attribute__((ext_vector_type(16)));
+
+template <int W> struct MfmaSpread2 {
+    static constexpr int T = 16;            // tile extent per axis
+    static constexpr int P = T + W - 1;     // padded extent per axis
+    static constexpr int NB = 32;           // nodes per batch
+    static constexpr int O_B = 32;          // table row: 32 row weights, then 64 columns (re, im)
+    static constexpr int ROW = 100;         // floats per table row (the two nodes a step reads start 4 banks apart)
+    static_assert(P <= 32 && O_B + 64 <= ROW, "a padded tile fits one 32-row product and a table row");
+};
+template <int W> constexpr bool kMfmaSpread2 = W >= 2 && 16 + W - 1 <= 32;
+
+template <int W>
+__device__ __forceinline__ float mfma2_weight(int win, float s, int m, float nrm, float kb, float shp)
+{
+    if (win == WIN_KAISER_BESSEL) return kb_phi<float>(s, (float)m, shp);
+    if (win == WIN_SINC) return sinc_phi<float>(s, m, shp);
+    return nrm * wexp<float>(-kb * s * s);
+}
+
+template <int W>
+__device__ __forceinline__ void tile_spread_mfma2(const NodeDesc<float> *__restrict__ desc,
+                                                  const cplx_t<float> *__restrict__ fs, const Chunk ch,
+                                                  cplx_t<float> *__restrict__ slot, const WinConst<float> &wc)
+{
+    using S = MfmaSpread2<W>;
+    using L = TileLayout<float, 2, W, 1, 1>;   // slot layout (shared with the fold)
+    static_assert(L::T0 == S::T && L::TL == S::T && L::P0 == S::P && L::PL == S::P && L::P1 == 1,
+                  "16^2 tiles");
+    __shared__ __align__(16) float tw[S::NB][S::ROW];
+    const int lane = threadIdx.x, nl = lane & 31, half = lane >> 5;
+    mf16_t acc0, acc1;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = 0.f;
+    for (uint32_t s0 = 0; s0 < ch.count; s0 += S::NB) {
+        const int nb = ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB;
+        if (s0 > 0) __syncthreads();   // the previous batch's steps have read the table
+        // lane (nl, half): half 0 writes node nl's row weights, half 1 its column weights
+        // times the coefficient; rows past the batch stay zero (a step reads two nodes)
+        float *row = &tw[nl][0] + (half == 0 ? 0 : S::O_B);
+        const int nz = half == 0 ? S::O_B / 4 : 16;   // 16-byte units of the lane's part
+        for (int u = 0; u < nz; ++u) reinterpret_cast<float4 *>(row)[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nl < nb) {
+            const NodeDesc<float> nd = desc[ch.start + s0 + nl];
+            const int t = half;   // axis
+            const int lo = (int)((nd.loc >> (8 * t)) & 255u);
+            const float ft = nd.f[t];
+            const float nrm = wc.nrm[t], kb = wc.kb[t], shp = wc.shp[t];
+            float fre = 1.f, fim = 0.f;
+            if (t == 1) {
+                const cplx_t<float> fv = fs[ch.start + s0 + nl];
+                fre = fv.x;
+                fim = fv.y;
+            }
+#pragma unroll
+            for (int l = 0; l < W; ++l) {
+                const float w = mfma2_weight<W>(wc.win, ft + (float)(wc.m - l), wc.m, nrm, kb, shp);
+                if (t == 0) {
+                    row[lo + l] = w;
+                } else {
+                    row[2 * (lo + l)] = w * fre;
+                    row[2 * (lo + l) + 1] = w * fim;
+                }
+            }
+        }
+        __syncthreads();
+        // two nodes per step: lane (nl, half) supplies node k + half's row nl and columns nl
+        for (int k = 0; k < nb; k += 2) {
+            const float *r = &tw[k + half][0];
+            const float a = r[nl];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, r[S::O_B + nl], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, r[S::O_B + 32 + nl], acc1, 0, 0, 0);
+        }
+    }
+    // column nl of block 0 is cell nl / 2 (part nl % 2), of block 1 cell 16 + nl / 2
+    float *out = reinterpret_cast<float *>(slot);
+    const int c0 = nl >> 1, ri = nl & 1, c1 = 16 + c0;
+    constexpr int EL1 = L::ext(L::TL, L::PL, 1);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int a = (i & 3) + 8 * (i >> 2) + 4 * half;
+        if (a < S::P) {
+            const int qa = a >> 4, ra = a & 15;
+            out[2 * (L::block_off(qa, 0, 0) + ra * S::T + c0) + ri] = acc0[i];
+            if (c1 < S::P) out[2 * (L::block_off(qa, 0, 1) + ra * EL1 + (c1 - 16)) + ri] = acc1[i];
+        }
+    }
+}
+
+// One wavefront per chunk of a plan's binning (blocks past the chunk count return).
+template <int W>
+__global__ __launch_bounds__(64) void k_tile_spread_mfma2(const NodeDesc<float> *__restrict__ desc,
+                                                          const cplx_t<float> *__restrict__ fs,
+                                                          const Chunk *__restrict__ chunks,
+                                                          const uint32_t *__restrict__ n_chunks,
+                                                          cplx_t<float> *__restrict__ slots, WinConst<float> wc)
+{
+    using L = TileLayout<float, 2, W, 1, 1>;
+    if (blockIdx.x >= n_chunks[0]) return;
+    tile_spread_mfma2<W>(desc, fs, chunks[blockIdx.x], slots + (int64_t)blockIdx.x * L::SC, wc);
+}
+
+// The batch form: workgroup b runs chunk map[b].blk of job map[b].job.
+template <int W>
+__global__ __launch_bounds__(64) void k_tile_spread_mfma2_batch(const GridJob<float> *__restrict__ jobs,
+                                                                const BlockMap *__restrict__ map)
+{
+    using L = TileLayout<float, 2, W, 1, 1>;
+    const BlockMap bm = map[blockIdx.x];
+    const GridJob<float> &J = jobs[bm.job];
+    tile_spread_mfma2<W>(static_cast<const NodeDesc<float> *>(J.desc), J.fs, J.chunks[bm.blk],
+                         J.slots + (int64_t)bm.blk * L::SC, J.wc);
+}
+
 }  // namespace cunfft

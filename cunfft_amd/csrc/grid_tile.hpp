@@ -2077,6 +2077,7 @@ template <typename T> struct SpreadOp {
             using L = TileLayout<T, D, W, NWS, V>;
             // the FP32 3D VALU spreads read f through the permutation themselves
             constexpr bool MFMA = D == 3 && V == 0 && kMfmaSpread<T, W>;
+            constexpr bool MFMA2 = sizeof(T) == 4 && D == 2 && V == 1 && kMfmaSpread2<W>;
             constexpr bool QUEUE = D == 3 && L::G == 1;
             constexpr bool PERM_F = QUEUE && !MFMA && sizeof(T) == 4;
             if (!PERM_F || b.compensated)
@@ -2108,6 +2109,10 @@ template <typename T> struct SpreadOp {
                 const unsigned grid = (unsigned)std::min<int64_t>(b.max_chunks, resident_blocks(km, NT));
                 km<<<grid, NT, 0, s>>>(static_cast<const NodeDesc<T> *>(b.desc), fs, b.chunks, b.n_chunks, slots,
                                        win_const<T>(ge));
+            } else if constexpr (MFMA2) {
+                k_tile_spread_mfma2<W><<<(unsigned)b.max_chunks, 64, 0, s>>>(
+                    reinterpret_cast<const NodeDesc<float> *>(b.desc), reinterpret_cast<const cplx_t<float> *>(fs),
+                    b.chunks, b.n_chunks, reinterpret_cast<cplx_t<float> *>(slots), win_const<float>(ge));
             } else if constexpr (QUEUE) {
                 // persistent held-unit spread: clear the work counter (a kernel, not a
                 // memset, so the launch sequence can be captured in a graph), then one
@@ -2152,7 +2157,11 @@ template <typename T> struct BatchOp {
                 k_sorted_coeffs_batch<T><<<nblk, 256, 0, s>>>(jobs, map);
                 return true;
             case BS_SPREAD:
-                k_tile_spread_batch<T, D, W, NWS, V><<<nblk, NWS * 64, 0, s>>>(jobs, map);
+                if constexpr (sizeof(T) == 4 && D == 2 && V == 1 && kMfmaSpread2<W>)
+                    k_tile_spread_mfma2_batch<W>
+                        <<<nblk, 64, 0, s>>>(reinterpret_cast<const GridJob<float> *>(jobs), map);
+                else
+                    k_tile_spread_batch<T, D, W, NWS, V><<<nblk, NWS * 64, 0, s>>>(jobs, map);
                 return true;
             case BS_FOLD:
                 k_tile_fold_batch<T, D, W, NWS, V><<<nblk, 256, 0, s>>>(jobs, map);

@@ -15,7 +15,10 @@ FP32-rounded inputs, to 1e-5 relative l2; FP64: to 1e-12):
   padding rows of each set);
 * the same chunk sums as the VALU spread of configuration 1 (8x8x16 tiles):
   both within the tolerance of the oracle and of each other;
-* a given binning gives a bit-identical grid call to call.
+* a given binning gives a bit-identical grid call to call;
+* 2D on 16^2 tiles (k_tile_spread_mfma2, configuration 1, FP32): m = 1..7, the
+  Kaiser-Bessel and sinc windows, a dense tile with odd-sized last batches,
+  agreement with the VALU spread of 32^2 tiles and bit-identical repeats.
 """
 import numpy as np
 import pytest
@@ -127,4 +130,51 @@ def test_agrees_with_the_valu_spread_and_repeats_bit_for_bit(gpu, orc, prec):
     print(f"{prec} matrix cores {ea:.3e}, VALU spread {ev:.3e}, between them {eav:.3e}")
     tol = TOL[prec]
     assert ea <= tol and ev <= tol and eav <= tol
+    assert np.array_equal(ha, hb)
+
+
+# ---- 2D on 16^2 tiles (configuration 1, FP32): one 32x32x2 product per two nodes
+
+def test_2d_every_window_width_matches_the_oracle(gpu, orc):
+    """m = 1..7 (W = 4..16: padded tiles of 19..31 rows, every one in one 32-row product)."""
+    N, n, M = [64, 64], [128, 128], 30000
+    for m in range(1, 8):
+        r = np.random.default_rng(600 + m)
+        x = r.uniform(-0.5, 0.5, (M, 2))
+        fv = crandn(r, M)
+        h = adjoint(gpu, "single", N, n, m, x, fv, config=1)
+        err = orc.rel_l2(orc.nfft_adjoint(N, n, m, x, rounded(fv, "single")), h)
+        print(f"2D m={m} W={2 * m + 2} rel-l2 vs oracle {err:.3e}")
+        assert err <= 1e-5
+
+
+@pytest.mark.parametrize("kind,flag", [("kaiser_bessel", "KAISER_BESSEL"), ("sinc", "SINC")])
+def test_2d_other_windows_match_the_oracle(gpu, orc, kind, flag):
+    N, n, m, M = [64, 64], [128, 128], 4, 30000
+    r = np.random.default_rng(620)
+    x = r.uniform(-0.5, 0.5, (M, 2))
+    fv = crandn(r, M)
+    h = adjoint(gpu, "single", N, n, m, x, fv, flags=getattr(gpu, flag), config=1)
+    with orc.window(kind):
+        ref = orc.nfft_adjoint(N, n, m, x, rounded(fv, "single"))
+    err = orc.rel_l2(ref, h)
+    print(f"2D {kind} rel-l2 vs oracle {err:.3e}")
+    assert err <= 1e-5
+
+
+def test_2d_dense_tile_odd_batches_and_agreement_with_the_valu_spread(gpu, orc):
+    """5001 nodes whose windows start in one 16^2 tile (several chunks, each batch of 32
+    but the last odd-sized) beside uniform ones; against the oracle and the VALU spread
+    of 32^2 tiles (configuration 0), and bit-identical call to call."""
+    N, n, m = [64, 64], [128, 128], 4
+    r = np.random.default_rng(640)
+    dense = (16 + m + r.uniform(-0.49, 15.49, (5001, 2))) / n[0]
+    x = np.concatenate([dense, r.uniform(-0.5, 0.5, (3333, 2))])
+    fv = crandn(r, x.shape[0])
+    ha, hb = adjoint(gpu, "single", N, n, m, x, fv, config=1, reps=2)
+    hv = adjoint(gpu, "single", N, n, m, x, fv, config=0)
+    ref = orc.nfft_adjoint(N, n, m, x, rounded(fv, "single"))
+    ea, ev, eav = orc.rel_l2(ref, ha), orc.rel_l2(ref, hv), orc.rel_l2(hv, ha)
+    print(f"2D matrix cores {ea:.3e}, VALU spread {ev:.3e}, between them {eav:.3e}")
+    assert ea <= 1e-5 and ev <= 1e-5 and eav <= 1e-5
     assert np.array_equal(ha, hb)

@@ -559,7 +559,8 @@ __global__ __launch_bounds__(kMfmaThreads<R>, kMfmaPerCu<R>) void k_tile_spread_
 // in-bounds probe).  Two 32-column blocks hold cells [0, 16) and [16, P).
 // One wavefront per chunk; per batch of 32 nodes the weights go to LDS in
 // padded-tile coordinates (zero outside the window), lanes n and n + 32
-// evaluating node n's row and column weights; then two MFMAs per two nodes.
+// evaluating node n's row and column weights (the batch's nodes loaded during
+// the previous batch's steps); then two MFMAs per two nodes.
 // The accumulators go straight to the slot: for a fixed register the 32
 // lanes of a half-wavefront store 32 consecutive floats of one row.
 typedef float mf16_t __attribute__((ext_vector_type(16)));
@@ -596,6 +597,13 @@ __device__ __forceinline__ void tile_spread_mfma2(const NodeDesc<float> *__restr
     mf16_t acc0, acc1;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = 0.f;
+    // lane (nl, half): node nl of the next batch (the coefficient on half 1 only)
+    NodeDesc<float> nd{};
+    cplx_t<float> fv = cmake<float>(0.f, 0.f);
+    if ((uint32_t)nl < ch.count) {
+        nd = desc[ch.start + nl];
+        if (half == 1) fv = fs[ch.start + nl];
+    }
     for (uint32_t s0 = 0; s0 < ch.count; s0 += S::NB) {
         const int nb = ch.count - s0 < (uint32_t)S::NB ? (int)(ch.count - s0) : S::NB;
         if (s0 > 0) __syncthreads();   // the previous batch's steps have read the table
@@ -605,17 +613,14 @@ __device__ __forceinline__ void tile_spread_mfma2(const NodeDesc<float> *__restr
         const int nz = half == 0 ? S::O_B / 4 : 16;   // 16-byte units of the lane's part
         for (int u = 0; u < nz; ++u) reinterpret_cast<float4 *>(row)[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (nl < nb) {
-            const NodeDesc<float> nd = desc[ch.start + s0 + nl];
             const int t = half;   // axis
-            const int lo = (int)((nd.loc >> (8 * t)) & 255u);
-            const float ft = nd.f[t];
+            // the node, loaded during the previous batch: opaque here, so its first use
+            // is not moved up to the load
+            uint32_t loc = nd.loc;
+            float ft = t == 0 ? nd.f[0] : nd.f[1], fre = t == 1 ? fv.x : 1.f, fim = t == 1 ? fv.y : 0.f;
+            asm volatile("" : "+v"(loc), "+v"(ft), "+v"(fre), "+v"(fim));
+            const int lo = (int)((loc >> (8 * t)) & 255u);
             const float nrm = wc.nrm[t], kb = wc.kb[t], shp = wc.shp[t];
-            float fre = 1.f, fim = 0.f;
-            if (t == 1) {
-                const cplx_t<float> fv = fs[ch.start + s0 + nl];
-                fre = fv.x;
-                fim = fv.y;
-            }
 #pragma unroll
             for (int l = 0; l < W; ++l) {
                 const float w = mfma2_weight<W>(wc.win, ft + (float)(wc.m - l), wc.m, nrm, kb, shp);
@@ -628,6 +633,11 @@ __device__ __forceinline__ void tile_spread_mfma2(const NodeDesc<float> *__restr
             }
         }
         __syncthreads();
+        // the next batch's node loads during this batch's steps
+        if (s0 + S::NB + nl < ch.count) {
+            nd = desc[ch.start + s0 + S::NB + nl];
+            if (half == 1) fv = fs[ch.start + s0 + S::NB + nl];
+        }
         // two nodes per step: lane (nl, half) supplies node k + half's row nl and columns nl
         for (int k = 0; k < nb; k += 2) {
             const float *r = &tw[k + half][0];

@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t *__res
                                                            uint32_t *__restrict__ vals_out)
 {
     __shared__ uint32_t base[kRsDigits];                        // next position per digit
-    __shared__ uint32_t wcnt[kRsThreads / 64][kRsDigits];       // this round's count per wavefront
+    __shared__ uint32_t wcnt[kRsThreads / 64][kRsDigits];       // this digit pass's count per wavefront
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     base[tid] = offs[(size_t)tid * nblk + blockIdx.x];
 #pragma unroll

@@ -18,7 +18,8 @@ FP32-rounded inputs, to 1e-5 relative l2; FP64: to 1e-12):
 * a given binning gives a bit-identical grid call to call;
 * 2D on 16^2 tiles (k_tile_spread_mfma2, configuration 1, FP32): m = 1..7, the
   Kaiser-Bessel and sinc windows, a dense tile with odd-sized last batches,
-  agreement with the VALU spread of 32^2 tiles and bit-identical repeats.
+  agreement with the VALU spread of 32^2 tiles and bit-identical repeats; the
+  batch's spread stage at m = 2, 6, 7 (matrix cores) and 8 (the VALU spread).
 """
 import numpy as np
 import pytest
@@ -178,3 +179,40 @@ def test_2d_dense_tile_odd_batches_and_agreement_with_the_valu_spread(gpu, orc):
     print(f"2D matrix cores {ea:.3e}, VALU spread {ev:.3e}, between them {eav:.3e}")
     assert ea <= 1e-5 and ev <= 1e-5 and eav <= 1e-5
     assert np.array_equal(ha, hb)
+
+
+@pytest.mark.parametrize("m", [2, 6, 7, 8])
+def test_2d_batch_spread_other_window_widths(gpu, orc, m):
+    """The batch's spread stage in configuration 1 (16^2 tiles): on the matrix cores
+    for m <= 7 (W <= 16, padded tiles of at most 31 rows), the VALU spread at m = 8
+    (33 rows); every (problem, right-hand side) against the oracle."""
+    import torch
+
+    from cunfft_amd.batch import Batch
+
+    r = np.random.default_rng(660 + m)
+    P, R = 5, 2
+    N = np.array([[int(r.choice([16, 24, 32, 48, 64])) for _ in range(2)] for _ in range(P)])
+    M = r.integers(500, 4000, P)
+    xs = [r.uniform(-0.5, 0.5, (int(M[i]), 2)) for i in range(P)]
+    fv = [crandn(r, R * int(M[i])).reshape(R, int(M[i])) for i in range(P)]
+    dev = torch.device("cuda")
+    with Batch(N, M, m=m, n_rhs=R, precision="single") as b:
+        b.config = 1
+        for i, x in enumerate(xs):
+            xt = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+            b.set_nodes_dev(i, xt.data_ptr())
+            _, _, f_in, _ = b.tensors(i)
+            f_in.copy_(torch.from_numpy(fv[i]).to(f_in.dtype))
+            torch.cuda.synchronize()
+        b.execute(gpu.TUNE_ADJOINT)
+        torch.cuda.synchronize()
+        worst = 0.0
+        for i in range(P):
+            out = b.tensors(i)[3].cpu().numpy().astype(np.complex128)
+            for k in range(R):
+                e = orc.rel_l2(orc.nfft_adjoint(list(N[i]), list(b.n[i]), m, xs[i],
+                                                rounded(fv[i][k], "single")), out[k])
+                worst = max(worst, e)
+                assert e <= 1e-5, (i, k, e)
+    print(f"2D batch m={m} worst rel-l2 vs oracle {worst:.3e}")
